@@ -1,0 +1,110 @@
+// common.h -- types shared by the host planner and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fftahip {
+
+constexpr int kMaxStages = 12;      // radix stages inside one in-LDS line FFT
+constexpr int kMaxLines = 256;      // lines per workgroup, at most
+// static LDS of a line-pass workgroup: five int64 per line
+constexpr int kLineSharedBytes = 5 * 8 * kMaxLines;
+
+// Unsigned 32-bit division by a runtime constant: q = (umulhi(n, m) + n) >> s.
+// Exact for every n < 2^32 when d < 2^31 (round-up method of Granlund and
+// Montgomery, 1994).
+struct FastDiv {
+  uint32_t d, m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  f.s = s;
+  uint64_t m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+  f.m = (uint32_t)m;
+  return f;
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, FastDiv f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t hi = __umulhi(n, f.m);
+#else
+  uint32_t hi = (uint32_t)(((uint64_t)n * f.m) >> 32);
+#endif
+  return (uint32_t)(((uint64_t)hi + n) >> f.s);
+}
+
+// What a line pass does when it loads a line into LDS.
+enum LoadMode : int32_t {
+  LOAD_PLAIN = 0,  // v = x
+  LOAD_CHIRP = 1,  // Bluestein: v = x[n] * conj(chirp[n]) for n < chirp_n, 0 beyond
+};
+
+// What a line pass does between its first and second radix sweep.
+enum MidMode : int32_t {
+  MID_NONE = 0,   // one sweep only
+  MID_BHAT = 1,   // Bluestein in one pass: v = conj(v * bhat[k]), then sweep again
+};
+
+// What a line pass does when it stores a line.
+enum StoreMode : int32_t {
+  STORE_PLAIN = 0,  // y = v
+  STORE_BHAT = 1,   // Bluestein spectrum product: y = conj(v * bhat[n])
+  STORE_CHIRP = 2,  // Bluestein end: y = conj(chirp[n] * v) for n < chirp_n only
+};
+
+// One batched pass of in-LDS line FFTs.  A "line" is L elements of a
+// transform; every line is independently transformed (forward sign), with
+// optional fused pre/post operations.  Lines are indexed (i0, i1, i2, i3)
+// with i0 fastest (extents n[0..2], i3 takes the rest); element t of a line
+// is read from
+//   in  + sum_d i_d*a[d] + (in_tab ? in_tab[i3] : 0) + t*ae
+// and output element k is written to
+//   out + sum_d i_d*b[d] + (out_tab ? out_tab[i3] : 0) + k*be.
+struct LinePass {
+  const void* in;
+  void* out;
+  int32_t L;
+  int32_t nst;
+  int32_t rad[kMaxStages];
+  int32_t twst[kMaxStages];       // L / (Ns * radix) per stage
+  FastDiv nsdiv[kMaxStages];      // by Ns (product of earlier radices) per stage
+  int32_t C;    // lines per workgroup
+  int32_t Tl;   // threads per line (C*Tl <= block, C <= kMaxLines)
+  int32_t ldl;  // LDS row pitch in elements (>= L)
+  int32_t col_in, col_out;  // 1: consecutive lines are adjacent in memory (coalesce across lines)
+  FastDiv divC, divL;       // by C and by L
+  int64_t nlines;
+  int64_t n[3];
+  int64_t a[4], b[4];
+  int64_t ae, be;
+  const int64_t* in_tab;
+  const int64_t* out_tab;
+  const void* tw;  // stage twiddles: tw[m] = exp(-2 pi i m / L), m < L, in the pass's precision
+  // Pre-twiddle (multi-pass Stockham): element t of a line is multiplied by
+  // w_N^{i_twdim * t * tw_scale}, w_N^m = lo[m & (2^sh-1)] * hi[m >> sh]
+  // (FP64 tables, product in FP64, rounded to the pass's precision).
+  int32_t pretw;
+  int32_t twdim;
+  int32_t tw_sh;
+  int32_t pad0_;
+  int64_t tw_scale;
+  const double2* tw_lo;
+  const double2* tw_hi;
+  int32_t conj_in, conj_out;
+  double scale;  // multiply on store when != 1
+  int32_t load_mode, mid_mode, store_mode;
+  int32_t xcd_remap;  // workgroup order: logical tile = (bid % 8) * (grid / 8) + bid / 8
+  // Chirp / spectrum index of a loaded element: n = sum_d i_d*ca[d] + t*cae;
+  // of a stored element: n = sum_d i_d*cb[d] + k*cbe.
+  int64_t chirp_n;
+  int64_t ca[4], cb[4];
+  int64_t cae, cbe;
+  const void* chirp;  // chirp_n entries, pass precision
+  const void* bhat;   // spectrum factor (mid: L entries; store: indexed by n)
+};
+
+}  // namespace fftahip

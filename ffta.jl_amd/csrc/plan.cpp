@@ -1,0 +1,1534 @@
+// plan.cpp -- host planner and C ABI of the transform library.
+//
+// The reference's planner (FFTA.jl src/callgraph.jl:76-126, CallGraphNode!)
+// classifies a length as radix-4 pow2 / pow3 / DFT / Bluestein / composite
+// and recurses.  Here the same classification is kept (it is what
+// fftahip_plan_describe reports and what the Julia wrapper exposes), and a
+// device schedule is emitted beside it:
+//   * lengths whose prime factors are all <= 13 run as self-sorting Stockham
+//     passes: one in-LDS pass when the whole line fits the workgroup, else a
+//     global mixed-radix Stockham over "pass factors" (each pass is a batch
+//     of in-LDS line FFTs with the inter-pass twiddle fused into its load);
+//   * any other length runs as Bluestein's convolution (src/algos.jl:344-424)
+//     over a power-of-two length M >= 2N-1: one kernel when M fits in LDS
+//     (chirp-multiply, FFT, spectrum product, FFT, chirp-multiply), else two
+//     multi-pass sweeps with the chirp fused into the first load, the
+//     spectrum product into the last store of the first sweep, and the final
+//     chirp into the last store.  The spectrum FFT(b)/M is computed once per
+//     plan in FP64 (FFTA recomputes it every call, src/algos.jl:409).
+// All tables (stage twiddles, two-level pass twiddles, chirp, spectrum) are
+// generated on the device in FP64 with exact integer angle reduction and
+// rounded to the plan's precision.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/fftahip.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace fftahip {
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+
+static fftahip_status fail(fftahip_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(FFTAHIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ device
+
+struct DevInfo {
+  int64_t lds = 65536;
+  int64_t cus = 256;
+  int64_t warp = 64;
+  bool ok = false;
+};
+
+static DevInfo& dev_info() {
+  static DevInfo info;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return;
+    info.lds = (int64_t)prop.sharedMemPerBlock;
+    info.cus = prop.multiProcessorCount;
+    info.warp = prop.warpSize;
+    info.ok = true;
+    init_kernels((size_t)info.lds);
+  });
+  return info;
+}
+
+// Dry-run mode (schedule inspection without a device): device pointers are
+// replaced by tokens (1<<60 | kind<<52 | offset) and nothing is launched.
+static thread_local bool g_dry = false;
+static thread_local int64_t g_dry_lds = 0;
+enum TokenKind { TK_STAGE = 1, TK_TWLO = 2, TK_TWHI = 3, TK_CHIRP = 4, TK_BHAT = 5, TK_IN = 6, TK_OUT = 7,
+                 TK_SCRATCH = 8, TK_OFFTAB = 10 };
+static void* token(int kind, int64_t v) {
+  return (void*)(uintptr_t)((1ull << 60) | ((uint64_t)kind << 52) | (uint64_t)v);
+}
+
+static int64_t env_i64(const char* name, int64_t dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return std::strtoll(v, nullptr, 10);
+}
+
+// ------------------------------------------------------------------ FFTA call graph
+// Restatement of src/callgraph.jl:76-126 for reporting (node list as FFTA
+// builds it: pre-order, relative child offsets, baked strides).
+
+enum FftaType { F_COMPOSITE = 0, F_DFT = 1, F_POW3 = 2, F_POW2RADIX4 = 3, F_BLUESTEIN = 4 };
+static const char* ffta_type_name(int t) {
+  switch (t) {
+    case F_COMPOSITE: return "COMPOSITE_FFT";
+    case F_DFT: return "DFT";
+    case F_POW3: return "POW3_FFT";
+    case F_POW2RADIX4: return "POW2RADIX4_FFT";
+    default: return "BLUESTEIN";
+  }
+}
+
+struct FftaNode {
+  int64_t left, right;
+  int type;
+  int64_t sz, s_in, s_out;
+};
+
+static std::vector<int64_t> prime_factors(int64_t n) {
+  std::vector<int64_t> f;
+  for (int64_t p = 2; p * p <= n; ++p)
+    while (n % p == 0) { f.push_back(p); n /= p; }
+  if (n > 1) f.push_back(n);
+  return f;
+}
+
+static bool is_prime(int64_t n) {
+  if (n < 2) return false;
+  for (int64_t p = 2; p * p <= n; ++p)
+    if (n % p == 0) return false;
+  return true;
+}
+
+static int64_t ffta_nodes(std::vector<FftaNode>& nodes, int64_t N, int64_t cutoff, int64_t s_in, int64_t s_out) {
+  if (N >= 2 && (N & (N - 1)) == 0) {
+    nodes.push_back({0, 0, F_POW2RADIX4, N, s_in, s_out});
+    return 1;
+  }
+  {
+    int64_t m = N;
+    bool pow3 = N >= 3 && N % 3 == 0;
+    while (pow3 && m % 3 == 0) m /= 3;
+    if (pow3 && m == 1) {
+      nodes.push_back({0, 0, F_POW3, N, s_in, s_out});
+      return 1;
+    }
+  }
+  if (N == 1 || is_prime(N)) {
+    nodes.push_back({0, 0, N < cutoff ? F_DFT : F_BLUESTEIN, N, s_in, s_out});
+    return 1;
+  }
+  std::vector<int64_t> Ns = prime_factors(N);
+  int64_t N1 = 1;
+  if (Ns[0] == 2 || Ns[0] == 3) {
+    for (int64_t p : Ns)
+      if (p == Ns[0]) N1 *= p;
+  } else {
+    const double fs = std::sqrt((double)N);
+    int64_t is = (int64_t)fs;
+    while (is * is > N) --is;
+    while ((is + 1) * (is + 1) <= N) ++is;
+    std::vector<int64_t> cp;
+    int64_t acc = 1;
+    for (int64_t p : Ns) { acc *= p; cp.push_back(acc); }
+    int64_t idx = 0;
+    for (size_t i = 0; i < cp.size(); ++i)
+      if (cp[i] <= is) idx = (int64_t)i;
+    N1 = cp[idx];
+    if (idx != (int64_t)cp.size() - 1 && ((double)cp[idx + 1] - fs < fs - (double)N1)) N1 = cp[idx + 1];
+  }
+  const int64_t N2 = N / N1;
+  const size_t at = nodes.size();
+  nodes.push_back({0, 0, F_DFT, N, s_in, s_out});
+  const int64_t l = ffta_nodes(nodes, N1, cutoff, N2, N2 * s_out);
+  const int64_t r = ffta_nodes(nodes, N2, cutoff, N1 * s_in, 1);
+  nodes[at] = {1, 1 + l, F_COMPOSITE, N, s_in, s_out};
+  return 1 + l + r;
+}
+
+// ------------------------------------------------------------------ device memory
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  fftahip_status ensure(size_t nbytes) {
+    if (nbytes <= bytes) return FFTAHIP_SUCCESS;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (nbytes == 0) return FFTAHIP_SUCCESS;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(FFTAHIP_ERR_ALLOC, std::string("hipMalloc(") + std::to_string(nbytes) + "): " + hipGetErrorString(e));
+    }
+    bytes = nbytes;
+    return FFTAHIP_SUCCESS;
+  }
+};
+
+// ------------------------------------------------------------------ in-LDS line FFT
+
+constexpr int kEPT = 16;
+
+struct Inner {
+  int64_t L = 1;
+  std::vector<int> rad;
+  int Tl = 1;
+  int block = 256;
+};
+
+// Radix schedule for an in-LDS line of length L (prime factors <= 13).
+static bool make_inner(int64_t L, Inner& in) {
+  in = Inner();
+  in.L = L;
+  int64_t m = L;
+  int e2 = 0, e3 = 0;
+  while (m % 2 == 0) { m /= 2; ++e2; }
+  while (m % 3 == 0) { m /= 3; ++e3; }
+  std::vector<int> rad;
+  for (int p : {13, 11, 7, 5})
+    while (m % p == 0) { m /= p; rad.push_back(p); }
+  if (m != 1) return false;
+  // powers of two: as few stages as possible, sizes balanced, each <= 16
+  if (e2 > 0) {
+    int ns = (e2 + 3) / 4;
+    int base = e2 / ns, extra = e2 % ns;
+    for (int i = 0; i < ns; ++i) rad.push_back(1 << (base + (i < extra ? 1 : 0)));
+  }
+  while (e3 >= 2) { rad.push_back(9); e3 -= 2; }
+  if (e3) rad.push_back(3);
+  std::sort(rad.begin(), rad.end(), [](int a, int b) { return a > b; });
+  if ((int)rad.size() > kMaxStages) return false;
+  int64_t Tl = 1;
+  for (int r : rad) {
+    const int64_t nb = kEPT / r;
+    const int64_t need = ((L / r) + nb - 1) / nb;
+    Tl = std::max(Tl, need);
+  }
+  if (Tl > 1024) return false;
+  in.rad = rad;
+  in.Tl = (int)Tl;
+  in.block = Tl <= 256 ? 256 : 1024;
+  return true;
+}
+
+static bool smooth13(int64_t n) {
+  for (int p : {2, 3, 5, 7, 11, 13})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+// ------------------------------------------------------------------ length plan
+
+enum Kind { K_SINGLE = 0, K_MULTI = 1, K_BLUE_SINGLE = 2, K_BLUE_MULTI = 3 };
+static const char* kind_name(int k) {
+  switch (k) {
+    case K_SINGLE: return "stockham-1pass";
+    case K_MULTI: return "stockham-multipass";
+    case K_BLUE_SINGLE: return "bluestein-1kernel";
+    default: return "bluestein-multipass";
+  }
+}
+
+struct Decomp {
+  int kind = K_SINGLE;
+  int64_t M = 0;                // device transform length (N, or Bluestein's M)
+  std::vector<int64_t> F;       // pass factors (product M); one entry for single-pass
+  std::vector<Inner> inner;     // per pass factor
+};
+
+struct Tables;  // per-plan device tables
+
+struct LengthPlan {
+  int prec = 32;
+  int64_t N = 1;
+  int64_t cutoff = 73;
+  std::vector<FftaNode> ffta;
+  bool blue = false;  // N has a prime factor > 13
+  int64_t M = 1;      // Bluestein length (pow2 >= 2N-1) or N
+  DevBuf chirp, bhat;
+  const void* chirp_ptr = nullptr;  // chirp.p, or a token in dry-run mode
+  const void* bhat_ptr = nullptr;
+  bool blue_ready = false;
+  std::map<int, Decomp> decomp;  // key: 0 contiguous lines, 1 strided lines
+};
+
+struct Tables {
+  int prec = 32;
+  std::map<int64_t, std::unique_ptr<DevBuf>> stage;               // L -> tw[L]
+  std::map<int64_t, std::pair<std::unique_ptr<DevBuf>, int>> two;  // M -> (lo|hi, sh)
+};
+
+static size_t csize(int prec) { return prec == 32 ? 8 : 16; }
+
+static fftahip_status stage_table(Tables& T, int64_t L, const void** out, hipStream_t s) {
+  if (g_dry) {
+    *out = token(TK_STAGE, L);
+    return FFTAHIP_SUCCESS;
+  }
+  auto it = T.stage.find(L);
+  if (it == T.stage.end()) {
+    auto buf = std::make_unique<DevBuf>();
+    fftahip_status st = buf->ensure((size_t)L * csize(T.prec));
+    if (st) return st;
+    HIP_TRY(launch_gen_stage_twiddles(buf->p, T.prec, L, s));
+    it = T.stage.emplace(L, std::move(buf)).first;
+  }
+  *out = it->second->p;
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status two_level(Tables& T, int64_t M, const double2** lo, const double2** hi, int* sh_out,
+                                hipStream_t s) {
+  if (g_dry) {
+    int bits = 0;
+    while ((1ll << bits) < M) ++bits;
+    *sh_out = (bits + 1) / 2;
+    *lo = (const double2*)token(TK_TWLO, M);
+    *hi = (const double2*)token(TK_TWHI, M);
+    return FFTAHIP_SUCCESS;
+  }
+  auto it = T.two.find(M);
+  if (it == T.two.end()) {
+    int bits = 0;
+    while ((1ll << bits) < M) ++bits;
+    const int sh = (bits + 1) / 2;
+    const int64_t nlo = 1ll << sh;
+    const int64_t nhi = (M + nlo - 1) / nlo;
+    auto buf = std::make_unique<DevBuf>();
+    fftahip_status st = buf->ensure((size_t)(nlo + nhi) * sizeof(double2));
+    if (st) return st;
+    double2* plo = (double2*)buf->p;
+    HIP_TRY(launch_gen_twiddle_2level(plo, plo + nlo, M, sh, nhi, s));
+    it = T.two.emplace(M, std::make_pair(std::move(buf), sh)).first;
+  }
+  const int sh = it->second.second;
+  double2* base = (double2*)it->second.first->p;
+  *lo = base;
+  *hi = base + (1ll << sh);
+  *sh_out = sh;
+  return FFTAHIP_SUCCESS;
+}
+
+// Largest in-LDS line for a contiguous line set, and the pass-factor cap for
+// strided line sets (keeps >= min_lines adjacent lines per workgroup so that
+// global accesses stay coalesced).
+static int64_t lds_budget() {
+  const int64_t cap = env_i64("FFTAHIP_LDS_BUDGET", 0);
+  int64_t lds = g_dry ? g_dry_lds : dev_info().lds;
+  if (lds < 65536) lds = 65536;
+  lds -= kLineSharedBytes + 1024;  // the kernel's static per-line tables
+  return cap > 0 ? std::min(cap, lds) : lds;
+}
+static int64_t min_lines(int prec) { return prec == 32 ? 8 : 4; }
+static int64_t single_max(int prec) {
+  int64_t m = lds_budget() / (int64_t)csize(prec);
+  return std::min<int64_t>(m, 1024 * kEPT);
+}
+static int64_t pass_max(int prec) {
+  const int64_t dflt = std::min<int64_t>(1024, lds_budget() / ((int64_t)csize(prec) * min_lines(prec)));
+  const int64_t v = env_i64("FFTAHIP_PASS_MAX", dflt);
+  return std::max<int64_t>(v, 16);
+}
+
+// Split M into pass factors, each <= cap and smooth; balanced greedily.
+static bool split_factors(int64_t M, int64_t cap, std::vector<int64_t>& F) {
+  std::vector<int64_t> primes = prime_factors(M);
+  std::sort(primes.begin(), primes.end(), std::greater<int64_t>());
+  for (int p = 2; p <= 8; ++p) {
+    std::vector<int64_t> g(p, 1);
+    bool ok = true;
+    for (int64_t q : primes) {
+      int best = -1;
+      for (int i = 0; i < p; ++i)
+        if (g[i] * q <= cap && (best < 0 || g[i] < g[best])) best = i;
+      if (best < 0) { ok = false; break; }
+      g[best] *= q;
+    }
+    if (!ok) continue;
+    F.clear();
+    for (int64_t x : g)
+      if (x > 1) F.push_back(x);
+    if (F.empty()) F.push_back(1);
+    // every factor must have an in-LDS schedule
+    bool sched = true;
+    Inner tmp;
+    for (int64_t x : F) sched = sched && make_inner(x, tmp);
+    if (sched) return true;
+  }
+  return false;
+}
+
+static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
+  d = Decomp();
+  const int prec = lp.prec;
+  const int64_t target = lp.blue ? lp.M : lp.N;
+  d.M = target;
+  const int64_t single_cap = strided ? pass_max(prec) : single_max(prec);
+  if (target <= single_cap) {
+    Inner in;
+    if (make_inner(target, in)) {
+      d.kind = lp.blue ? K_BLUE_SINGLE : K_SINGLE;
+      d.F = {target};
+      d.inner = {in};
+      return FFTAHIP_SUCCESS;
+    }
+  }
+  std::vector<int64_t> F;
+  if (!split_factors(target, pass_max(prec), F))
+    return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(target));
+  d.kind = lp.blue ? K_BLUE_MULTI : K_MULTI;
+  d.F = F;
+  for (int64_t f : F) {
+    Inner in;
+    make_inner(f, in);
+    d.inner.push_back(in);
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status get_decomp(LengthPlan& lp, int strided, Decomp** out) {
+  auto it = lp.decomp.find(strided);
+  if (it == lp.decomp.end()) {
+    Decomp d;
+    fftahip_status st = make_decomp(lp, strided, d);
+    if (st) return st;
+    it = lp.decomp.emplace(strided, d).first;
+  }
+  *out = &it->second;
+  return FFTAHIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ line sets and passes
+
+// A set of independent lines of one length: up to two affine batch dims, or a
+// table of offsets (ragged batches).  Positions within a line are scaled by
+// es_in / es_out.
+struct LineSet {
+  int64_t n[2] = {1, 1};
+  int64_t in_str[2] = {0, 0};
+  int64_t out_str[2] = {0, 0};
+  const int64_t* in_tab = nullptr;   // device; per-line offsets (then n = {1, count})
+  const int64_t* out_tab = nullptr;
+  int64_t es_in = 1, es_out = 1;
+  int64_t count() const { return n[0] * n[1]; }
+};
+
+struct Dim {
+  int64_t n, a, b, ca, cb;
+  bool tw;
+};
+
+enum LaunchOp { OP_PASS = 0, OP_COPY = 1 };
+struct Launch {
+  int op = OP_PASS;
+  LinePass lp;
+  int block = 256;
+  size_t lds = 0;
+  int64_t grid = 0;
+  size_t bytes = 0;  // OP_COPY
+};
+
+// Fill geometry: dims (<= 3 affine + optional table dim) -> LinePass fields.
+static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
+  // unit dims carry nothing; the table dim (if any) stays last
+  std::vector<Dim> D;
+  for (size_t i = 0; i < Din.size(); ++i)
+    if (Din[i].n > 1 || (tab && i + 1 == Din.size())) D.push_back(Din[i]);
+  if (D.empty()) D.push_back(Dim{1, 0, 0, 0, 0, false});
+  // choose slot 0 among the affine dims for coalescing
+  const size_t naff = tab ? D.size() - 1 : D.size();
+  int pick = -1;
+  for (size_t i = 0; i < naff && pick < 0; ++i)
+    if (D[i].n > 1 && D[i].a == 1 && p.ae != 1) pick = (int)i;
+  for (size_t i = 0; i < naff && pick < 0; ++i)
+    if (D[i].n > 1 && D[i].b == 1 && p.be != 1) pick = (int)i;
+  if (pick > 0) std::swap(D[0], D[pick]);
+  std::vector<Dim> slots(4, Dim{1, 0, 0, 0, 0, false});
+  for (size_t i = 0; i < naff; ++i) slots[i] = D[i];
+  if (tab) slots[3] = D.back();
+  int64_t total = 1;
+  for (int i = 0; i < 4; ++i) total *= slots[i].n;
+  p.nlines = total;
+  for (int i = 0; i < 3; ++i) p.n[i] = slots[i].n;
+  p.twdim = 0;
+  for (int i = 0; i < 4; ++i) {
+    p.a[i] = slots[i].a;
+    p.b[i] = slots[i].b;
+    p.ca[i] = slots[i].ca;
+    p.cb[i] = slots[i].cb;
+    if (slots[i].tw) p.twdim = i;
+  }
+  p.col_in = (p.ae != 1 && slots[0].n > 1 && slots[0].a == 1) ? 1 : 0;
+  p.col_out = (p.be != 1 && slots[0].n > 1 && slots[0].b == 1) ? 1 : 0;
+}
+
+static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
+  p.L = (int32_t)in.L;
+  p.nst = (int32_t)in.rad.size();
+  int64_t Ns = 1;
+  for (int s = 0; s < kMaxStages; ++s) {
+    if (s < p.nst) {
+      const int r = in.rad[s];
+      p.rad[s] = r;
+      p.twst[s] = (int32_t)(in.L / (Ns * r));
+      p.nsdiv[s] = make_fastdiv((uint32_t)Ns);
+      Ns *= r;
+    } else {
+      p.rad[s] = 1;
+      p.twst[s] = 0;
+      p.nsdiv[s] = make_fastdiv(1);
+    }
+  }
+  p.Tl = in.Tl;
+  const int64_t esz = (int64_t)csize(prec);
+  int64_t C = std::min<int64_t>(kMaxLines, std::max<int64_t>(1, in.block / in.Tl));
+  const int64_t budget = lds_budget();
+  int64_t ldl = in.L + ((p.col_in || p.col_out) ? 1 : 0);
+  while (C > 1 && C * ldl * esz > budget) --C;
+  if (C > p.nlines) C = std::max<int64_t>(1, p.nlines);
+  if (C == 1) ldl = in.L;
+  p.C = (int32_t)C;
+  p.ldl = (int32_t)ldl;
+  p.divC = make_fastdiv((uint32_t)C);
+  p.divL = make_fastdiv((uint32_t)in.L);
+  L.block = in.block;
+  L.lds = (size_t)(C * ldl * esz);
+  L.grid = (p.nlines + C - 1) / C;
+  p.xcd_remap = (L.grid >= 16 && L.grid % 8 == 0) ? 1 : 0;
+}
+
+static LinePass blank_pass() {
+  LinePass p;
+  std::memset(&p, 0, sizeof(p));
+  p.scale = 1.0;
+  p.ae = p.be = 1;
+  p.n[0] = p.n[1] = p.n[2] = 1;
+  return p;
+}
+
+// Build the launches that transform every line of `ls` through `lp`.
+// conj: backward sign (conjugate in and out).
+static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& ls, const void* in, void* out,
+                                     void* s0, void* s1, bool conj, std::vector<Launch>& L, hipStream_t stream) {
+  const int strided = (ls.es_in != 1 || ls.es_out != 1) ? 1 : 0;
+  Decomp* d;
+  fftahip_status st = get_decomp(lp, strided, &d);
+  if (st) return st;
+  const int prec = lp.prec;
+  const int64_t M = d->M;
+  const int64_t B = ls.count();
+  const bool tab = ls.in_tab != nullptr;
+
+  if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE) {
+    Launch la;
+    LinePass p = blank_pass();
+    p.in = in;
+    p.out = out;
+    p.ae = ls.es_in;
+    p.be = ls.es_out;
+    std::vector<Dim> D;
+    if (tab) {
+      D.push_back(Dim{ls.n[1], 0, 0, 0, 0, false});
+      p.in_tab = ls.in_tab;
+      p.out_tab = ls.out_tab;
+    } else {
+      for (int i = 0; i < 2; ++i)
+        if (ls.n[i] > 1) D.push_back(Dim{ls.n[i], ls.in_str[i], ls.out_str[i], 0, 0, false});
+      if (D.empty()) D.push_back(Dim{1, 0, 0, 0, 0, false});
+    }
+    set_dims(p, D, tab);
+    st = stage_table(T, d->F[0], &p.tw, stream);
+    if (st) return st;
+    p.conj_in = conj ? 1 : 0;
+    p.conj_out = conj ? 1 : 0;
+    if (d->kind == K_BLUE_SINGLE) {
+      p.load_mode = LOAD_CHIRP;
+      p.mid_mode = MID_BHAT;
+      p.store_mode = STORE_CHIRP;
+      p.chirp_n = lp.N;
+      p.cae = 1;
+      p.cbe = 1;
+      p.chirp = lp.chirp_ptr;
+      p.bhat = lp.bhat_ptr;
+    }
+    set_inner(p, d->inner[0], prec, la);
+    la.lp = p;
+    L.push_back(la);
+    return FFTAHIP_SUCCESS;
+  }
+
+  // multi-pass (Stockham over pass factors), possibly two sweeps (Bluestein)
+  const int np = (int)d->F.size();
+  const bool blue = d->kind == K_BLUE_MULTI;
+  const int total = blue ? 2 * np : np;
+  const double2* tlo = nullptr;
+  const double2* thi = nullptr;
+  int tsh = 0;
+  st = two_level(T, M, &tlo, &thi, &tsh, stream);
+  if (st) return st;
+  for (int i = 0; i < total; ++i) {
+    const int s = i % np;
+    const int64_t R = d->F[s];
+    int64_t Ns = 1;
+    for (int u = 0; u < s; ++u) Ns *= d->F[u];
+    const int64_t Q = M / (R * Ns);
+    const bool first = i == 0, last = i == total - 1;
+    Launch la;
+    LinePass p = blank_pass();
+    p.in = first ? in : (((i - 1) % 2 == 0) ? s0 : s1);
+    p.out = last ? out : ((i % 2 == 0) ? s0 : s1);
+    const int64_t ein = first ? ls.es_in : 1;
+    const int64_t eout = last ? ls.es_out : 1;
+    p.ae = ein * (M / R);
+    p.be = eout * Ns;
+    std::vector<Dim> D;
+    // k (position within the Ns group) and q (group index)
+    D.push_back(Dim{Ns, ein, eout, 1, 1, true});
+    D.push_back(Dim{Q, ein * Ns, eout * Ns * R, Ns, Ns * R, false});
+    // batch: user side uses the line-set layout, scratch side b*M
+    if (tab) {
+      Dim bd{ls.n[1], first ? 0 : M, last ? 0 : M, 0, 0, false};
+      D.push_back(bd);
+      p.in_tab = first ? ls.in_tab : nullptr;
+      p.out_tab = last ? ls.out_tab : nullptr;
+    } else {
+      int64_t sstride = M;
+      for (int b = 0; b < 2; ++b) {
+        if (ls.n[b] > 1) {
+          D.push_back(Dim{ls.n[b], first ? ls.in_str[b] : sstride, last ? ls.out_str[b] : sstride, 0, 0, false});
+        }
+        sstride *= ls.n[b];
+      }
+    }
+    set_dims(p, D, tab);
+    st = stage_table(T, R, &p.tw, stream);
+    if (st) return st;
+    if (Ns > 1) {
+      p.pretw = 1;
+      p.tw_scale = M / (Ns * R);
+      p.tw_lo = tlo;
+      p.tw_hi = thi;
+      p.tw_sh = tsh;
+    }
+    p.conj_in = (first && conj) ? 1 : 0;
+    p.conj_out = (last && conj) ? 1 : 0;
+    if (blue) {
+      p.chirp_n = lp.N;
+      p.chirp = lp.chirp_ptr;
+      p.bhat = lp.bhat_ptr;
+      p.cae = M / R;
+      p.cbe = Ns;
+      if (first) p.load_mode = LOAD_CHIRP;
+      if (i == np - 1) p.store_mode = STORE_BHAT;
+      if (last) p.store_mode = STORE_CHIRP;
+    }
+    set_inner(p, d->inner[s], prec, la);
+    la.lp = p;
+    L.push_back(la);
+  }
+  (void)B;
+  return FFTAHIP_SUCCESS;
+}
+
+// Scratch elements per line needed by build_launches.
+static fftahip_status scratch_per_line(LengthPlan& lp, int strided, int64_t* elems, int* nbuf) {
+  Decomp* d;
+  fftahip_status st = get_decomp(lp, strided, &d);
+  if (st) return st;
+  *elems = d->M;
+  if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE) *nbuf = 0;
+  else if (d->kind == K_MULTI && d->F.size() == 2) *nbuf = 1;
+  else *nbuf = 2;
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status run_launches(const std::vector<Launch>& L, int prec, hipStream_t s) {
+  for (const Launch& la : L) {
+    if (la.op == OP_COPY) {
+      HIP_TRY(hipMemcpyAsync(la.lp.out, la.lp.in, la.bytes, hipMemcpyDeviceToDevice, s));
+      continue;
+    }
+    if (la.grid <= 0) continue;
+    if (la.grid > 0x7fffffffLL) return fail(FFTAHIP_ERR_CAPACITY, "grid too large");
+    HIP_TRY(launch_line_pass(la.lp, prec, la.block, la.lds, la.grid, s));
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ Bluestein spectrum
+
+// bhat = FFT_M(b_ext) / M computed with an FP64 plan of the same machinery.
+static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s) {
+  const int64_t N = lp.N, M = lp.M;
+  fftahip_status st = lp.chirp.ensure((size_t)N * csize(lp.prec));
+  if (st) return st;
+  st = lp.bhat.ensure((size_t)M * csize(lp.prec));
+  if (st) return st;
+  DevBuf c64, bext, spec, s0, s1;
+  if ((st = c64.ensure((size_t)N * sizeof(double2)))) return st;
+  if ((st = bext.ensure((size_t)M * sizeof(double2)))) return st;
+  if ((st = spec.ensure((size_t)M * sizeof(double2)))) return st;
+  HIP_TRY(launch_gen_chirp(lp.chirp.p, (double2*)c64.p, lp.prec, N, s));
+  HIP_TRY(launch_gen_chirp_ext((double2*)bext.p, (const double2*)c64.p, N, M, s));
+  LengthPlan sub;
+  sub.prec = 64;
+  sub.N = M;
+  sub.M = M;
+  sub.blue = false;
+  Tables T64;
+  T64.prec = 64;
+  int64_t per;
+  int nbuf;
+  if ((st = scratch_per_line(sub, 0, &per, &nbuf))) return st;
+  if (nbuf >= 1 && (st = s0.ensure((size_t)per * 16))) return st;
+  if (nbuf >= 2 && (st = s1.ensure((size_t)per * 16))) return st;
+  LineSet ls;
+  std::vector<Launch> L;
+  if ((st = build_launches(sub, T64, ls, bext.p, spec.p, s0.p, s1.p, false, L, s))) return st;
+  if ((st = run_launches(L, 64, s))) return st;
+  HIP_TRY(launch_finish_bhat(lp.bhat.p, (const double2*)spec.p, lp.prec, M, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  lp.chirp_ptr = lp.chirp.p;
+  lp.bhat_ptr = lp.bhat.p;
+  lp.blue_ready = true;
+  (void)T;
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t cutoff) {
+  lp.prec = prec;
+  lp.N = N;
+  lp.cutoff = cutoff;
+  lp.ffta.clear();
+  ffta_nodes(lp.ffta, N, cutoff, 1, 1);
+  lp.blue = !smooth13(N);
+  if (lp.blue) {
+    int64_t M = 1;
+    while (M < 2 * N - 1) M <<= 1;
+    lp.M = M;
+  } else {
+    lp.M = N;
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status prepare_length(LengthPlan& lp, Tables& T, hipStream_t s) {
+  if (lp.blue && g_dry) {
+    lp.chirp_ptr = token(TK_CHIRP, lp.N);
+    lp.bhat_ptr = token(TK_BHAT, lp.M);
+    lp.blue_ready = true;
+    return FFTAHIP_SUCCESS;
+  }
+  if (lp.blue && !lp.blue_ready) return prepare_bluestein(lp, T, s);
+  return FFTAHIP_SUCCESS;
+}
+
+}  // namespace fftahip
+
+using namespace fftahip;
+
+// ------------------------------------------------------------------ plans
+
+struct fftahip_plan_s {
+  int prec = 32;
+  int dir = -1;
+  int64_t cutoff = 73;
+  std::vector<int64_t> region;                    // sorted, 1-based
+  std::vector<std::unique_ptr<LengthPlan>> lens;  // one per region dim
+  Tables tables;
+  DevBuf scratch;
+  DevBuf hin, hout;  // staging for execute_host
+  std::string last_schedule;
+};
+
+struct fftahip_batch_s {
+  int prec = 32;
+  int64_t count = 0;
+  std::vector<int64_t> lengths, offsets;
+  std::vector<int> dirs;
+  int64_t total = 0;  // elements spanned
+  std::map<int64_t, std::unique_ptr<LengthPlan>> lens;
+  Tables tables;
+  struct Group {
+    int64_t N;
+    int dir;
+    std::vector<int64_t> idx;  // transform indices
+  };
+  std::vector<Group> groups;
+  DevBuf offtab;   // per group offsets (device), concatenated
+  DevBuf scratch;  // per group scratch, concatenated
+  const int64_t* offtab_ptr = nullptr;  // offtab.p, or a token in dry-run mode
+  char* scratch_ptr = nullptr;
+  std::vector<int64_t> host_offtab;
+  std::vector<std::vector<Launch>> per_group;  // built launches (dry-run inspection)
+  DevBuf passes;   // device LinePass arrays per launch
+  DevBuf tiles;
+  struct Phase {
+    int block;
+    size_t lds;
+    int64_t grid;
+    int ngroups;
+    size_t pass_off, tile_off;  // element offsets into passes / tiles
+  };
+  std::vector<Phase> phases;
+  std::vector<LinePass> host_passes;
+  std::vector<uint32_t> host_tiles;
+  const void* built_in = nullptr;
+  void* built_out = nullptr;
+  hipGraphExec_t graph = nullptr;
+  const void* graph_in = nullptr;
+  void* graph_out = nullptr;
+  ~fftahip_batch_s() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+  }
+};
+
+extern "C" {
+
+const char* fftahip_version(void) { return "fftahip 0.1.0 (gfx950)"; }
+
+const char* fftahip_status_string(fftahip_status s) {
+  switch (s) {
+    case FFTAHIP_SUCCESS: return "success";
+    case FFTAHIP_ERR_DIMENSION_MISMATCH: return "DimensionMismatch";
+    case FFTAHIP_ERR_ARGUMENT: return "ArgumentError";
+    case FFTAHIP_ERR_DOMAIN: return "DomainError";
+    case FFTAHIP_ERR_HIP: return "HIP runtime error";
+    case FFTAHIP_ERR_ALLOC: return "allocation failed";
+    case FFTAHIP_ERR_NOT_IMPLEMENTED: return "not implemented";
+    case FFTAHIP_ERR_CAPACITY: return "capacity exceeded";
+    case FFTAHIP_ERR_NOT_READY: return "not ready";
+  }
+  return "unknown status";
+}
+
+const char* fftahip_last_error(void) { return g_err.c_str(); }
+
+fftahip_status fftahip_plan_create(fftahip_plan* plan, int precision, int direction, int ndims, const int64_t* dims,
+                                   int nregion, const int64_t* region, int64_t bluestein_cutoff) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "plan pointer is NULL");
+  *plan = nullptr;
+  if (precision != 32 && precision != 64) return fail(FFTAHIP_ERR_ARGUMENT, "precision must be 32 or 64");
+  if (direction != -1 && direction != 1) return fail(FFTAHIP_ERR_ARGUMENT, "direction must be -1 or +1");
+  if (nregion < 1) return fail(FFTAHIP_ERR_ARGUMENT, "region must name at least one dimension");
+  std::vector<int64_t> reg(region, region + nregion);
+  std::sort(reg.begin(), reg.end());
+  auto p = std::make_unique<fftahip_plan_s>();
+  p->prec = precision;
+  p->dir = direction;
+  p->cutoff = bluestein_cutoff;
+  p->region = reg;
+  p->tables.prec = precision;
+  for (int64_t r : reg) {
+    if (r < 1) return fail(FFTAHIP_ERR_ARGUMENT, "arraysize: dimension out of range");
+    const int64_t n = r <= ndims ? dims[r - 1] : 1;  // size(x, r) is 1 beyond ndims
+    if (n <= 0) return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Array length must be strictly positive");
+    auto lp = std::make_unique<LengthPlan>();
+    init_length(*lp, precision, n, bluestein_cutoff);
+    p->lens.push_back(std::move(lp));
+  }
+  hipStream_t s = nullptr;
+  for (auto& lp : p->lens) {
+    fftahip_status st = prepare_length(*lp, p->tables, s);
+    if (st) return st;
+  }
+  *plan = p.release();
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_plan_size(fftahip_plan plan, int i, int64_t* size_out) {
+  if (!plan || !size_out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  if (i < 1) return fail(FFTAHIP_ERR_DOMAIN, "No non-positive dimensions");
+  if (i > (int)plan->lens.size()) {
+    *size_out = 1;
+    return FFTAHIP_SUCCESS;
+  }
+  *size_out = plan->lens[i - 1]->N;
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_plan_rank(fftahip_plan plan, int* rank_out) {
+  if (!plan || !rank_out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  *rank_out = (int)plan->lens.size();
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status fftahip_plan_precision(fftahip_plan plan, int* out) {
+  if (!plan || !out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  *out = plan->prec;
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status fftahip_plan_direction(fftahip_plan plan, int* out) {
+  if (!plan || !out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  *out = plan->dir;
+  return FFTAHIP_SUCCESS;
+}
+
+static std::string tuple_str(const int64_t* d, int n) {
+  std::ostringstream os;
+  os << "(";
+  for (int i = 0; i < n; ++i) {
+    if (i) os << ", ";
+    os << d[i];
+  }
+  if (n == 1) os << ",";
+  os << ")";
+  return os.str();
+}
+
+static std::string axes_str(const int64_t* d, int n) {
+  std::ostringstream os;
+  os << "(";
+  for (int i = 0; i < n; ++i) {
+    if (i) os << ", ";
+    os << "Base.OneTo(" << d[i] << ")";
+  }
+  if (n == 1) os << ",";
+  os << ")";
+  return os.str();
+}
+
+fftahip_status fftahip_plan_check(fftahip_plan plan, int nregion, const int64_t* region, int ndims_x,
+                                  const int64_t* dims_x, int ndims_y, const int64_t* dims_y) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  const int M = (int)plan->lens.size();
+  std::vector<int64_t> psize(M);
+  for (int i = 0; i < M; ++i) psize[i] = plan->lens[i]->N;
+  auto same = [](int na, const int64_t* a, int nb, const int64_t* b) {
+    if (na != nb) return false;
+    for (int i = 0; i < na; ++i)
+      if (a[i] != b[i]) return false;
+    return true;
+  };
+  const bool axes_equal = same(ndims_x, dims_x, ndims_y, dims_y);
+  if (M == 1 && ndims_x == 1 && ndims_y == 1) {
+    // src/plan.jl:157-164
+    if (!axes_equal)
+      return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "input array has axes " + axes_str(dims_x, 1) +
+                                                      ", but output array has axes " + axes_str(dims_y, 1));
+    if (psize[0] != dims_x[0])
+      return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "plan has axes " + tuple_str(psize.data(), 1) +
+                                                      ", but input array has axes " + tuple_str(dims_x, 1));
+    return FFTAHIP_SUCCESS;
+  }
+  if (M == 1) {
+    // src/plan.jl:170-182
+    if (!axes_equal)
+      return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "input array has axes " + axes_str(dims_x, ndims_x) +
+                                                      ", but output array has axes " + axes_str(dims_y, ndims_y));
+    const int64_t R1 = nregion >= 1 ? region[0] : plan->region[0];
+    const int64_t xlen = (R1 >= 1 && R1 <= ndims_x) ? dims_x[R1 - 1] : 1;
+    if (psize[0] != xlen)
+      return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "plan has size " + std::to_string(psize[0]) +
+                                                      ", but input array has size " + std::to_string(xlen) +
+                                                      " along region " + std::to_string(R1));
+    return FFTAHIP_SUCCESS;
+  }
+  const int N = ndims_x;
+  if (!same(ndims_x, dims_x, ndims_y, dims_y))
+    return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "input array has axes " + axes_str(dims_x, ndims_x) +
+                                                    ", but output array has axes " + axes_str(dims_y, ndims_y));
+  if (M == N) {
+    // src/plan.jl:216-222
+    if (!same(M, psize.data(), N, dims_x))
+      return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "plan has size " + tuple_str(psize.data(), M) +
+                                                      ", but input array has size " + tuple_str(dims_x, N));
+    bool valid = nregion == N;
+    for (int i = 0; valid && i < nregion; ++i) valid = region[i] == i + 1;
+    if (!valid) return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Plan region is outside array dimensions.");
+    return FFTAHIP_SUCCESS;
+  }
+  // src/plan.jl:266-272 (also reached for M > N)
+  bool valid = nregion == M;
+  for (int i = 1; valid && i < nregion; ++i) valid = region[i - 1] < region[i];
+  if (!valid) return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Region is invalid.");
+  if (M > N || region[0] < 1 || region[nregion - 1] > N)
+    return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Plan region is outside array dimensions.");
+  for (int i = 0; i < M; ++i)
+    if (dims_x[region[i] - 1] != psize[i])
+      return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "plan has size " + tuple_str(psize.data(), M) +
+                                                      ", but input array has size " + tuple_str(dims_x, N));
+  return FFTAHIP_SUCCESS;
+}
+
+// Lines along dimension d (0-based) of a column-major array.
+static LineSet lines_along(int ndims, const int64_t* dims, int d) {
+  int64_t pre = 1, post = 1;
+  for (int i = 0; i < d && i < ndims; ++i) pre *= dims[i];
+  for (int i = d + 1; i < ndims; ++i) post *= dims[i];
+  const int64_t n = d < ndims ? dims[d] : 1;
+  LineSet ls;
+  ls.es_in = ls.es_out = pre;
+  ls.n[0] = pre;
+  ls.in_str[0] = ls.out_str[0] = 1;
+  ls.n[1] = post;
+  ls.in_str[1] = ls.out_str[1] = pre * n;
+  return ls;
+}
+
+// The launches of mul!(y, p, x) for an array of shape dims: region dims in
+// ascending order, the first reading x and every later one working in y
+// (src/plan.jl:232-253).  `scratch` must hold *scratch_need bytes.
+static fftahip_status plan_schedule(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
+                                    void* scratch, std::vector<Launch>& L, size_t* scratch_need, hipStream_t s) {
+  int64_t total = 1;
+  for (int i = 0; i < ndims; ++i) total *= dims[i];
+  size_t need = 0;
+  for (size_t k = 0; k < plan->region.size(); ++k) {
+    const int d = (int)plan->region[k] - 1;
+    if (d >= ndims) continue;
+    LineSet ls = lines_along(ndims, dims, d);
+    int64_t per;
+    int nbuf;
+    fftahip_status st = scratch_per_line(*plan->lens[k], (ls.es_in != 1) ? 1 : 0, &per, &nbuf);
+    if (st) return st;
+    need = std::max(need, (size_t)(per * ls.count() * nbuf) * csize(plan->prec));
+  }
+  *scratch_need = need;
+  if (!scratch && need > 0) return FFTAHIP_SUCCESS;  // sizing only
+  bool wrote = false;  // d_out already holds the partial result
+  for (size_t k = 0; k < plan->region.size(); ++k) {
+    const int d = (int)plan->region[k] - 1;
+    if (d >= ndims) {
+      // size(x, d) == 1 beyond ndims: a length-1 transform is the identity
+      if (!wrote && d_in != d_out) {
+        Launch cp;
+        cp.op = OP_COPY;
+        cp.lp = blank_pass();
+        cp.lp.in = d_in;
+        cp.lp.out = d_out;
+        cp.bytes = (size_t)total * csize(plan->prec);
+        L.push_back(cp);
+      }
+      wrote = true;
+      continue;
+    }
+    LineSet ls = lines_along(ndims, dims, d);
+    int64_t per;
+    int nbuf;
+    fftahip_status st = scratch_per_line(*plan->lens[k], (ls.es_in != 1) ? 1 : 0, &per, &nbuf);
+    if (st) return st;
+    char* s0 = (char*)scratch;
+    char* s1 = nbuf >= 2 ? s0 + (size_t)(per * ls.count()) * csize(plan->prec) : nullptr;
+    const void* src = wrote ? (const void*)d_out : d_in;
+    wrote = true;
+    st = build_launches(*plan->lens[k], plan->tables, ls, src, d_out, s0, s1, plan->dir == 1, L, s);
+    if (st) return st;
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
+                               void* stream) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  hipStream_t s = (hipStream_t)stream;
+  int64_t total = 1;
+  for (int i = 0; i < ndims; ++i) total *= dims[i];
+  if (total == 0) return FFTAHIP_SUCCESS;
+  std::vector<Launch> L;
+  size_t need = 0;
+  fftahip_status st = plan_schedule(plan, ndims, dims, d_in, d_out, nullptr, L, &need, s);
+  if (st) return st;
+  if ((st = plan->scratch.ensure(need))) return st;
+  L.clear();
+  if ((st = plan_schedule(plan, ndims, dims, d_in, d_out, plan->scratch.p, L, &need, s))) return st;
+  return run_launches(L, plan->prec, s);
+}
+
+fftahip_status fftahip_execute_host(fftahip_plan plan, int ndims, const int64_t* dims, const void* h_in,
+                                    void* h_out) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  int64_t total = 1;
+  for (int i = 0; i < ndims; ++i) total *= dims[i];
+  const size_t bytes = (size_t)total * csize(plan->prec);
+  if (bytes == 0) return FFTAHIP_SUCCESS;
+  fftahip_status st;
+  if ((st = plan->hin.ensure(bytes))) return st;
+  if ((st = plan->hout.ensure(bytes))) return st;
+  HIP_TRY(hipMemcpy(plan->hin.p, h_in, bytes, hipMemcpyHostToDevice));
+  if ((st = fftahip_execute(plan, ndims, dims, plan->hin.p, plan->hout.p, nullptr))) return st;
+  HIP_TRY(hipMemcpy(h_out, plan->hout.p, bytes, hipMemcpyDeviceToHost));
+  return FFTAHIP_SUCCESS;
+}
+
+static void describe_length(std::ostringstream& os, LengthPlan& lp) {
+  os << "{\"n\": " << lp.N << ", \"ffta\": [";
+  for (size_t i = 0; i < lp.ffta.size(); ++i) {
+    const FftaNode& q = lp.ffta[i];
+    if (i) os << ", ";
+    os << "{\"type\": \"" << ffta_type_name(q.type) << "\", \"sz\": " << q.sz << ", \"s_in\": " << q.s_in
+       << ", \"s_out\": " << q.s_out << ", \"left\": " << q.left << ", \"right\": " << q.right << "}";
+  }
+  os << "], \"device\": {";
+  bool firstk = true;
+  for (int strided = 0; strided < 2; ++strided) {
+    Decomp* d;
+    if (get_decomp(lp, strided, &d) != FFTAHIP_SUCCESS) continue;
+    if (!firstk) os << ", ";
+    firstk = false;
+    os << "\"" << (strided ? "strided" : "contiguous") << "\": {\"kind\": \"" << kind_name(d->kind)
+       << "\", \"m\": " << d->M << ", \"passes\": [";
+    for (size_t i = 0; i < d->F.size(); ++i) {
+      if (i) os << ", ";
+      os << "{\"L\": " << d->F[i] << ", \"radices\": [";
+      for (size_t r = 0; r < d->inner[i].rad.size(); ++r) os << (r ? ", " : "") << d->inner[i].rad[r];
+      os << "], \"threads_per_line\": " << d->inner[i].Tl << ", \"block\": " << d->inner[i].block << "}";
+    }
+    os << "]}";
+  }
+  os << "}}";
+}
+
+static fftahip_status emit(const std::string& s, char* buf, size_t cap, size_t* needed) {
+  if (needed) *needed = s.size() + 1;
+  if (buf && cap > 0) {
+    const size_t n = std::min(cap - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_plan_describe(fftahip_plan plan, char* buf, size_t cap, size_t* needed) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  std::ostringstream os;
+  os << "{\"precision\": " << plan->prec << ", \"direction\": " << plan->dir << ", \"lds_budget\": " << lds_budget()
+     << ", \"region\": [";
+  for (size_t i = 0; i < plan->region.size(); ++i) os << (i ? ", " : "") << plan->region[i];
+  os << "], \"dims\": [";
+  for (size_t i = 0; i < plan->lens.size(); ++i) {
+    if (i) os << ", ";
+    describe_length(os, *plan->lens[i]);
+  }
+  os << "]}";
+  return emit(os.str(), buf, cap, needed);
+}
+
+fftahip_status fftahip_plan_destroy(fftahip_plan plan) {
+  if (plan) {
+    (void)hipDeviceSynchronize();
+    delete plan;
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ batches
+
+fftahip_status fftahip_batch_create(fftahip_batch* batch, int precision, int64_t count, const int64_t* lengths,
+                                    const int64_t* offsets, const int* directions, int64_t bluestein_cutoff) {
+  if (!batch) return fail(FFTAHIP_ERR_ARGUMENT, "batch pointer is NULL");
+  *batch = nullptr;
+  if (precision != 32 && precision != 64) return fail(FFTAHIP_ERR_ARGUMENT, "precision must be 32 or 64");
+  if (count < 0) return fail(FFTAHIP_ERR_ARGUMENT, "count must be >= 0");
+  auto b = std::make_unique<fftahip_batch_s>();
+  b->prec = precision;
+  b->count = count;
+  b->tables.prec = precision;
+  b->lengths.assign(lengths, lengths + count);
+  b->dirs.assign(directions, directions + count);
+  b->offsets.resize(count);
+  int64_t acc = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    if (lengths[i] <= 0) return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Array length must be strictly positive");
+    if (directions[i] != -1 && directions[i] != 1) return fail(FFTAHIP_ERR_ARGUMENT, "direction must be -1 or +1");
+    b->offsets[i] = offsets ? offsets[i] : acc;
+    acc += lengths[i];
+    b->total = std::max(b->total, b->offsets[i] + lengths[i]);
+  }
+  // group by (length, direction)
+  std::map<std::pair<int64_t, int>, size_t> gi;
+  for (int64_t i = 0; i < count; ++i) {
+    auto key = std::make_pair(b->lengths[i], b->dirs[i]);
+    auto it = gi.find(key);
+    if (it == gi.end()) {
+      it = gi.emplace(key, b->groups.size()).first;
+      b->groups.push_back({b->lengths[i], b->dirs[i], {}});
+    }
+    b->groups[it->second].idx.push_back(i);
+  }
+  hipStream_t s = nullptr;
+  for (auto& g : b->groups) {
+    if (!b->lens.count(g.N)) {
+      auto lp = std::make_unique<LengthPlan>();
+      init_length(*lp, precision, g.N, bluestein_cutoff);
+      fftahip_status st = prepare_length(*lp, b->tables, s);
+      if (st) return st;
+      b->lens.emplace(g.N, std::move(lp));
+    }
+  }
+  // offset tables and scratch
+  std::vector<int64_t> offs;
+  size_t scratch_elems = 0;
+  for (auto& g : b->groups) {
+    for (int64_t i : g.idx) offs.push_back(b->offsets[i]);
+    int64_t per;
+    int nbuf;
+    fftahip_status st = scratch_per_line(*b->lens[g.N], 0, &per, &nbuf);
+    if (st) return st;
+    scratch_elems += (size_t)(per * (int64_t)g.idx.size() * nbuf);
+  }
+  b->host_offtab = offs;
+  if (g_dry) {
+    b->offtab_ptr = (const int64_t*)token(TK_OFFTAB, 0);
+    b->scratch_ptr = (char*)token(TK_SCRATCH, 0);
+    b->scratch.bytes = scratch_elems * csize(precision);
+    *batch = b.release();
+    return FFTAHIP_SUCCESS;
+  }
+  fftahip_status st = b->offtab.ensure(offs.size() * sizeof(int64_t));
+  if (st) return st;
+  if (!offs.empty()) HIP_TRY(hipMemcpy(b->offtab.p, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  if ((st = b->scratch.ensure(scratch_elems * csize(precision)))) return st;
+  b->offtab_ptr = (const int64_t*)b->offtab.p;
+  b->scratch_ptr = (char*)b->scratch.p;
+  HIP_TRY(hipDeviceSynchronize());
+  *batch = b.release();
+  return FFTAHIP_SUCCESS;
+}
+
+// Build the per-phase work lists for the given pointers.
+static fftahip_status batch_build(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
+  b->phases.clear();
+  b->host_passes.clear();
+  b->host_tiles.clear();
+  std::vector<std::vector<Launch>>& per_group = b->per_group;
+  per_group.clear();
+  size_t toff = 0;
+  char* scr = b->scratch_ptr;
+  const size_t es = csize(b->prec);
+  for (auto& g : b->groups) {
+    LengthPlan& lp = *b->lens[g.N];
+    LineSet ls;
+    ls.n[0] = 1;
+    ls.n[1] = (int64_t)g.idx.size();
+    ls.in_tab = b->offtab_ptr + toff;
+    ls.out_tab = ls.in_tab;
+    toff += g.idx.size();
+    int64_t per;
+    int nbuf;
+    fftahip_status st = scratch_per_line(lp, 0, &per, &nbuf);
+    if (st) return st;
+    void* s0 = nbuf >= 1 ? scr : nullptr;
+    void* s1 = nbuf >= 2 ? scr + (size_t)(per * ls.n[1]) * es : nullptr;
+    scr += (size_t)(per * ls.n[1] * nbuf) * es;
+    std::vector<Launch> L;
+    st = build_launches(lp, b->tables, ls, d_in, d_out, s0, s1, g.dir == 1, L, s);
+    if (st) return st;
+    per_group.push_back(L);
+  }
+  // phase p = the p-th launch of every group; within a phase, one grouped
+  // launch per block size.
+  size_t maxp = 0;
+  for (auto& L : per_group) maxp = std::max(maxp, L.size());
+  for (size_t ph = 0; ph < maxp; ++ph) {
+    for (int block : {256, 1024}) {
+      fftahip_batch_s::Phase P;
+      P.block = block;
+      P.lds = 0;
+      P.ngroups = 0;
+      P.pass_off = b->host_passes.size();
+      P.tile_off = b->host_tiles.size();
+      uint32_t tiles = 0;
+      for (auto& L : per_group) {
+        if (ph >= L.size() || L[ph].block != block || L[ph].grid <= 0) continue;
+        LinePass lp = L[ph].lp;
+        lp.xcd_remap = 0;
+        b->host_passes.push_back(lp);
+        b->host_tiles.push_back(tiles);
+        tiles += (uint32_t)L[ph].grid;
+        P.lds = std::max(P.lds, L[ph].lds);
+        ++P.ngroups;
+      }
+      if (P.ngroups == 0) continue;
+      P.grid = tiles;
+      b->phases.push_back(P);
+    }
+  }
+  if (g_dry) {
+    b->built_in = d_in;
+    b->built_out = d_out;
+    return FFTAHIP_SUCCESS;
+  }
+  fftahip_status st = b->passes.ensure(b->host_passes.size() * sizeof(LinePass));
+  if (st) return st;
+  if ((st = b->tiles.ensure(b->host_tiles.size() * sizeof(uint32_t)))) return st;
+  if (!b->host_passes.empty())
+    HIP_TRY(hipMemcpy(b->passes.p, b->host_passes.data(), b->host_passes.size() * sizeof(LinePass),
+                      hipMemcpyHostToDevice));
+  if (!b->host_tiles.empty())
+    HIP_TRY(hipMemcpy(b->tiles.p, b->host_tiles.data(), b->host_tiles.size() * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+  b->built_in = d_in;
+  b->built_out = d_out;
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status batch_launch(fftahip_batch b, hipStream_t s) {
+  for (auto& P : b->phases) {
+    HIP_TRY(launch_line_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
+                                    P.ngroups, b->prec, P.block, P.lds, P.grid, s));
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_batch_execute(fftahip_batch b, const void* d_in, void* d_out, void* stream) {
+  if (!b) return fail(FFTAHIP_ERR_ARGUMENT, "NULL batch");
+  hipStream_t s = (hipStream_t)stream;
+  if (b->count == 0) return FFTAHIP_SUCCESS;
+  if (b->built_in != d_in || b->built_out != d_out || b->phases.empty()) {
+    HIP_TRY(hipStreamSynchronize(s));
+    fftahip_status st = batch_build(b, d_in, d_out, s);
+    if (st) return st;
+  }
+  return batch_launch(b, s);
+}
+
+fftahip_status fftahip_batch_execute_graph(fftahip_batch b, const void* d_in, void* d_out, void* stream) {
+  if (!b) return fail(FFTAHIP_ERR_ARGUMENT, "NULL batch");
+  hipStream_t s = (hipStream_t)stream;
+  if (b->count == 0) return FFTAHIP_SUCCESS;
+  if (b->built_in != d_in || b->built_out != d_out || b->phases.empty()) {
+    HIP_TRY(hipStreamSynchronize(s));
+    fftahip_status st = batch_build(b, d_in, d_out, s);
+    if (st) return st;
+    if (b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+  }
+  if (!b->graph) {
+    hipStream_t cs;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    fftahip_status st = batch_launch(b, cs);
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (st) return st;
+    if (e != hipSuccess) return fail(FFTAHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      b->graph = nullptr;
+      return fail(FFTAHIP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(hipGraphLaunch(b->graph, s));
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_batch_describe(fftahip_batch b, char* buf, size_t cap, size_t* needed) {
+  if (!b) return fail(FFTAHIP_ERR_ARGUMENT, "NULL batch");
+  std::ostringstream os;
+  os << "{\"precision\": " << b->prec << ", \"count\": " << b->count << ", \"groups\": " << b->groups.size()
+     << ", \"lengths\": " << b->lens.size() << ", \"phases\": [";
+  for (size_t i = 0; i < b->phases.size(); ++i) {
+    auto& P = b->phases[i];
+    os << (i ? ", " : "") << "{\"block\": " << P.block << ", \"lds\": " << P.lds << ", \"grid\": " << P.grid
+       << ", \"groups\": " << P.ngroups << "}";
+  }
+  os << "], \"scratch_bytes\": " << b->scratch.bytes << "}";
+  return emit(os.str(), buf, cap, needed);
+}
+
+fftahip_status fftahip_batch_destroy(fftahip_batch b) {
+  if (b) {
+    (void)hipDeviceSynchronize();
+    delete b;
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ helpers
+
+fftahip_status fftahip_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(FFTAHIP_ERR_ARGUMENT, "NULL pointer");
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return fail(FFTAHIP_ERR_ALLOC, hipGetErrorString(e));
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status fftahip_free(void* ptr) {
+  HIP_TRY(hipFree(ptr));
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status fftahip_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status fftahip_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status fftahip_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status fftahip_device_info(int64_t* lds_per_block, int64_t* cu_count, int64_t* warp_size) {
+  DevInfo& d = dev_info();
+  if (!d.ok) return fail(FFTAHIP_ERR_HIP, "no HIP device");
+  if (lds_per_block) *lds_per_block = d.lds;
+  if (cu_count) *cu_count = d.cus;
+  if (warp_size) *warp_size = d.warp;
+  return FFTAHIP_SUCCESS;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ schedule inspection
+// Dry-run dumps used by the CPU test-suite to check pass geometry (index
+// maps, twiddle exponents, buffer bounds) without a device.
+
+static void json_ptr(std::ostringstream& os, const char* name, const void* p) {
+  const uint64_t v = (uint64_t)(uintptr_t)p;
+  if ((v >> 60) == 1) {
+    os << "\"" << name << "\": [" << ((v >> 52) & 0xff) << ", " << (v & ((1ull << 52) - 1)) << "]";
+  } else {
+    os << "\"" << name << "\": " << (p ? "[-1, 0]" : "null");
+  }
+}
+
+static void json_launch(std::ostringstream& os, const Launch& la) {
+  const LinePass& p = la.lp;
+  os << "{\"op\": " << la.op << ", \"block\": " << la.block << ", \"lds\": " << la.lds << ", \"grid\": " << la.grid
+     << ", \"bytes\": " << la.bytes << ", ";
+  json_ptr(os, "in", p.in);
+  os << ", ";
+  json_ptr(os, "out", p.out);
+  os << ", \"L\": " << p.L << ", \"rad\": [";
+  for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.rad[i];
+  os << "], \"twst\": [";
+  for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.twst[i];
+  os << "], \"C\": " << p.C << ", \"Tl\": " << p.Tl << ", \"ldl\": " << p.ldl << ", \"col_in\": " << p.col_in
+     << ", \"col_out\": " << p.col_out << ", \"nlines\": " << p.nlines << ", \"n\": [" << p.n[0] << ", " << p.n[1]
+     << ", " << p.n[2] << "], \"a\": [" << p.a[0] << ", " << p.a[1] << ", " << p.a[2] << ", " << p.a[3]
+     << "], \"b\": [" << p.b[0] << ", " << p.b[1] << ", " << p.b[2] << ", " << p.b[3] << "], \"ae\": " << p.ae
+     << ", \"be\": " << p.be << ", ";
+  json_ptr(os, "in_tab", p.in_tab);
+  os << ", ";
+  json_ptr(os, "out_tab", p.out_tab);
+  os << ", ";
+  json_ptr(os, "tw", p.tw);
+  os << ", \"pretw\": " << p.pretw << ", \"twdim\": " << p.twdim << ", \"tw_sh\": " << p.tw_sh
+     << ", \"tw_scale\": " << p.tw_scale << ", ";
+  json_ptr(os, "tw_lo", p.tw_lo);
+  os << ", ";
+  json_ptr(os, "tw_hi", p.tw_hi);
+  os << ", \"conj_in\": " << p.conj_in << ", \"conj_out\": " << p.conj_out << ", \"scale\": " << p.scale
+     << ", \"load_mode\": " << p.load_mode << ", \"mid_mode\": " << p.mid_mode << ", \"store_mode\": " << p.store_mode
+     << ", \"xcd_remap\": " << p.xcd_remap << ", \"chirp_n\": " << p.chirp_n << ", \"ca\": [" << p.ca[0] << ", "
+     << p.ca[1] << ", " << p.ca[2] << ", " << p.ca[3] << "], \"cb\": [" << p.cb[0] << ", " << p.cb[1] << ", "
+     << p.cb[2] << ", " << p.cb[3] << "], \"cae\": " << p.cae << ", \"cbe\": " << p.cbe << ", ";
+  json_ptr(os, "chirp", p.chirp);
+  os << ", ";
+  json_ptr(os, "bhat", p.bhat);
+  os << "}";
+}
+
+struct DryGuard {
+  DryGuard(int64_t lds) {
+    g_dry = true;
+    g_dry_lds = lds;
+  }
+  ~DryGuard() {
+    g_dry = false;
+    g_dry_lds = 0;
+  }
+};
+
+extern "C" {
+
+// Schedule of mul!(y, plan_fft(x, region), x) for an array of shape dims on a
+// device with `lds_per_block` bytes of LDS, as JSON: {"scratch_bytes", "launches"}.
+fftahip_status fftahip_debug_schedule(int precision, int direction, int ndims, const int64_t* dims, int nregion,
+                                      const int64_t* region, int64_t lds_per_block, char* buf, size_t cap,
+                                      size_t* needed) {
+  DryGuard g(lds_per_block);
+  fftahip_plan plan = nullptr;
+  fftahip_status st = fftahip_plan_create(&plan, precision, direction, ndims, dims, nregion, region, 73);
+  if (st) return st;
+  std::unique_ptr<fftahip_plan_s> owner(plan);
+  std::vector<Launch> L;
+  size_t need = 0;
+  st = plan_schedule(plan, ndims, dims, token(TK_IN, 0), token(TK_OUT, 0), token(TK_SCRATCH, 0), L, &need, nullptr);
+  if (st) return st;
+  std::ostringstream os;
+  os << "{\"scratch_bytes\": " << need << ", \"launches\": [";
+  for (size_t i = 0; i < L.size(); ++i) {
+    if (i) os << ", ";
+    json_launch(os, L[i]);
+  }
+  os << "]}";
+  return emit(os.str(), buf, cap, needed);
+}
+
+// Grouped schedule of a ragged batch: {"scratch_bytes", "offtab", "groups": [[launch...]...]}.
+fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const int64_t* lengths,
+                                            const int* directions, int64_t lds_per_block, char* buf, size_t cap,
+                                            size_t* needed) {
+  DryGuard g(lds_per_block);
+  fftahip_batch b = nullptr;
+  fftahip_status st = fftahip_batch_create(&b, precision, count, lengths, nullptr, directions, 73);
+  if (st) return st;
+  std::unique_ptr<fftahip_batch_s> owner(b);
+  if ((st = batch_build(b, token(TK_IN, 0), token(TK_OUT, 0), nullptr))) return st;
+  std::ostringstream os;
+  os << "{\"scratch_bytes\": " << b->scratch.bytes << ", \"offtab\": [";
+  for (size_t i = 0; i < b->host_offtab.size(); ++i) os << (i ? ", " : "") << b->host_offtab[i];
+  os << "], \"groups\": [";
+  for (size_t gi = 0; gi < b->per_group.size(); ++gi) {
+    os << (gi ? ", " : "") << "[";
+    for (size_t i = 0; i < b->per_group[gi].size(); ++i) {
+      if (i) os << ", ";
+      json_launch(os, b->per_group[gi][i]);
+    }
+    os << "]";
+  }
+  os << "], \"phases\": " << b->phases.size() << "}";
+  return emit(os.str(), buf, cap, needed);
+}
+
+// Host evaluation of the kernels' division by a runtime constant.
+fftahip_status fftahip_debug_fastdiv(uint32_t d, const uint32_t* n, uint32_t* q, int64_t count) {
+  if (d == 0) return fail(FFTAHIP_ERR_ARGUMENT, "division by zero");
+  const FastDiv f = make_fastdiv(d);
+  for (int64_t i = 0; i < count; ++i) q[i] = fdiv(n[i], f);
+  return FFTAHIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+"""fftahip -- FFTA.jl's complex fft/bfft plan interface on MI355X (gfx950).
+
+Host-side mirror of FFTA.jl's AbstractFFTs plan boundary (src/plan.jl),
+calling the C ABI of libfftahip.so through ctypes.  Arrays follow Julia's
+conventions: shapes and regions are Julia's (1-based dims, column-major
+memory).  numpy arrays are taken as column-major (Fortran order) data;
+device arrays are passed as (pointer, dims).
+
+Reference correspondence:
+  plan_fft / plan_bfft      src/plan.jl:71-100
+  Plan.size(i)              src/plan.jl:36-47
+  mul_(y, p, x)             LinearAlgebra.mul!, src/plan.jl:157-285
+  p * x                     Base.:*, src/plan.jl:374-384
+  fft / bfft / ifft         AbstractFFTs convenience front-ends (plan, then apply;
+                            ifft = bfft scaled by 1/prod(size) over the region)
+Errors map to DimensionMismatch, ArgumentError and DomainError with the
+reference's messages.
+"""
+from .plan import (  # noqa: F401
+    FFT_FORWARD,
+    FFT_BACKWARD,
+    DEFAULT_BLUESTEIN_CUTOFF,
+    ArgumentError,
+    DimensionMismatch,
+    DomainError,
+    FFTAHipError,
+    Plan,
+    Batch,
+    plan_fft,
+    plan_bfft,
+    fft,
+    bfft,
+    ifft,
+    mul_,
+    device_info,
+    version,
+)

@@ -1,0 +1,153 @@
+/* fftahip.h -- C ABI of the MI355X (gfx950) transform library that replaces
+ * FFTA.jl's complex fft/bfft path.
+ *
+ * FFTA.jl has no FFI: its boundary is the AbstractFFTs plan interface in
+ * src/plan.jl.  Each entry point below names the reference function it
+ * replaces (paths relative to the FFTA.jl source tree).  A Julia wrapper
+ * reaches these through ccall; the Python package in ffta.jl_amd/fftahip
+ * reaches them through ctypes.
+ *
+ * Conventions
+ *  - Arrays are column-major, 1-based regions (as in Julia), interleaved
+ *    complex: ComplexF32 = 2 x float (precision 32), ComplexF64 = 2 x double
+ *    (precision 64).
+ *  - direction: -1 = FFT_FORWARD (exp(-2 pi i jk/N)), +1 = FFT_BACKWARD
+ *    (unnormalised inverse), as src/callgraph.jl:1.
+ *  - Device pointers are plain pointers to device memory; `stream` is a
+ *    hipStream_t passed as void* (NULL = the default stream).
+ *  - Every call returns an fftahip_status; on failure fftahip_last_error()
+ *    returns a thread-local message (the reference's exception text where
+ *    the reference defines one).
+ *  - A plan owns device scratch and tables; one execution of a given plan may
+ *    be in flight at a time (FFTA plans share a workspace in the same way,
+ *    src/callgraph.jl:40).  Plans are independent of each other.
+ */
+#ifndef FFTAHIP_H
+#define FFTAHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  FFTAHIP_SUCCESS = 0,
+  FFTAHIP_ERR_DIMENSION_MISMATCH = 1, /* Julia DimensionMismatch */
+  FFTAHIP_ERR_ARGUMENT = 2,           /* Julia ArgumentError */
+  FFTAHIP_ERR_DOMAIN = 3,             /* Julia DomainError */
+  FFTAHIP_ERR_HIP = 4,                /* a HIP runtime call failed */
+  FFTAHIP_ERR_ALLOC = 5,              /* device or host allocation failed */
+  FFTAHIP_ERR_NOT_IMPLEMENTED = 6,
+  FFTAHIP_ERR_CAPACITY = 7,           /* request exceeds a configured budget */
+  FFTAHIP_ERR_NOT_READY = 8           /* serving: result not yet available */
+} fftahip_status;
+
+#define FFTAHIP_FORWARD (-1)
+#define FFTAHIP_BACKWARD 1
+#define FFTAHIP_C32 32
+#define FFTAHIP_C64 64
+#define FFTAHIP_DEFAULT_BLUESTEIN_CUTOFF 73
+
+typedef struct fftahip_plan_s* fftahip_plan;
+
+/* Library version string. */
+const char* fftahip_version(void);
+/* Text of a status code. */
+const char* fftahip_status_string(fftahip_status s);
+/* Message of the last failed call on this thread ("" if none). */
+const char* fftahip_last_error(void);
+
+/* plan_fft / plan_bfft (src/plan.jl:71-100, _plan_fft).
+ * x has `ndims` dimensions of sizes dims[0..ndims-1] (column-major); region
+ * lists the 1-based dimensions to transform (it is sorted, as _sort does).
+ * One call graph is planned per region dimension (CallGraph{T}(size(x,r),
+ * BLUESTEIN_CUTOFF), src/callgraph.jl:133-138); a length <= 0 fails with
+ * DimensionMismatch("Array length must be strictly positive")
+ * (src/callgraph.jl:83-85).  bluestein_cutoff mirrors the BLUESTEIN_CUTOFF
+ * keyword (src/callgraph.jl:44); it selects FFTA's tree (reported by
+ * fftahip_plan_describe) -- results agree either way within tolerance. */
+fftahip_status fftahip_plan_create(fftahip_plan* plan, int precision, int direction, int ndims, const int64_t* dims,
+                                   int nregion, const int64_t* region, int64_t bluestein_cutoff);
+
+/* Base.size(p, i) (src/plan.jl:36-47): i < 1 -> DomainError, i > M -> 1. */
+fftahip_status fftahip_plan_size(fftahip_plan plan, int i, int64_t* size_out);
+/* Number of transformed dimensions M (the N of FFTAPlan_cx{T,N}). */
+fftahip_status fftahip_plan_rank(fftahip_plan plan, int* rank_out);
+fftahip_status fftahip_plan_precision(fftahip_plan plan, int* precision_out);
+fftahip_status fftahip_plan_direction(fftahip_plan plan, int* direction_out);
+
+/* The argument checks of LinearAlgebra.mul!(y, p, x) (src/plan.jl:157-164,
+ * 170-182, 210-222, 260-272), with the plan's region as the caller holds it
+ * (`region`, nregion entries; FFTA plans carry a mutable region).  Returns
+ * FFTAHIP_ERR_DIMENSION_MISMATCH with the reference's message when the arrays
+ * do not fit the plan. */
+fftahip_status fftahip_plan_check(fftahip_plan plan, int nregion, const int64_t* region, int ndims_x,
+                                  const int64_t* dims_x, int ndims_y, const int64_t* dims_y);
+
+/* mul!(y, p, x) on device arrays of shape dims (src/plan.jl:157-167 for 1-D,
+ * 170-207 for a 1-D plan on an N-D array, 210-257 and 260-325 for N-D).
+ * Out of place; d_in may equal d_out.  Asynchronous on `stream`. */
+fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
+                               void* stream);
+
+/* The same on host arrays: copies in, transforms, copies out, synchronises.
+ * This is the drop-in path for Julia Array arguments. */
+fftahip_status fftahip_execute_host(fftahip_plan plan, int ndims, const int64_t* dims, const void* h_in,
+                                    void* h_out);
+
+/* JSON description: FFTA's call-graph per region dimension (node type, size,
+ * strides) and the device schedule (passes, radices, tiles). */
+fftahip_status fftahip_plan_describe(fftahip_plan plan, char* buf, size_t cap, size_t* needed);
+
+fftahip_status fftahip_plan_destroy(fftahip_plan plan);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent 1-D transforms of uneven lengths (the batched use of
+ * fft!(out, in, ...) per vector, src/algos.jl:5-34).  Transform i reads
+ * lengths[i] elements at element offset offsets[i] of d_in and writes the same
+ * range of d_out; directions[i] is -1 or +1.  offsets may be NULL (packed:
+ * prefix sums of lengths). */
+typedef struct fftahip_batch_s* fftahip_batch;
+
+fftahip_status fftahip_batch_create(fftahip_batch* batch, int precision, int64_t count, const int64_t* lengths,
+                                    const int64_t* offsets, const int* directions, int64_t bluestein_cutoff);
+fftahip_status fftahip_batch_execute(fftahip_batch batch, const void* d_in, void* d_out, void* stream);
+/* Captures one execution into a hipGraph on first use and replays it on
+ * later calls with the same pointers. */
+fftahip_status fftahip_batch_execute_graph(fftahip_batch batch, const void* d_in, void* d_out, void* stream);
+fftahip_status fftahip_batch_describe(fftahip_batch batch, char* buf, size_t cap, size_t* needed);
+fftahip_status fftahip_batch_destroy(fftahip_batch batch);
+
+/* ---------------------------------------------------------------- device helpers
+ * Minimal memory helpers so that a host language without a GPU array type
+ * (the Julia wrapper, plain C) can run the device entry points. */
+fftahip_status fftahip_malloc(void** ptr, size_t bytes);
+fftahip_status fftahip_free(void* ptr);
+fftahip_status fftahip_memcpy_htod(void* dst, const void* src, size_t bytes);
+fftahip_status fftahip_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+fftahip_status fftahip_synchronize(void);
+/* Device properties the planner uses: LDS bytes per workgroup, CU count. */
+fftahip_status fftahip_device_info(int64_t* lds_per_block, int64_t* cu_count, int64_t* warp_size);
+
+/* ---------------------------------------------------------------- schedule inspection
+ * Dry runs of the planner (no device needed): the launches that
+ * fftahip_execute / fftahip_batch_execute would issue on a device with
+ * `lds_per_block` bytes of LDS, as JSON with device pointers replaced by
+ * [kind, offset] tokens.  No reference counterpart; used by the test-suite to
+ * check pass geometry on the host. */
+fftahip_status fftahip_debug_schedule(int precision, int direction, int ndims, const int64_t* dims, int nregion,
+                                      const int64_t* region, int64_t lds_per_block, char* buf, size_t cap,
+                                      size_t* needed);
+fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const int64_t* lengths,
+                                            const int* directions, int64_t lds_per_block, char* buf, size_t cap,
+                                            size_t* needed);
+/* Host evaluation of the kernels' division by a runtime constant d. */
+fftahip_status fftahip_debug_fastdiv(uint32_t d, const uint32_t* n, uint32_t* q, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFTAHIP_H */

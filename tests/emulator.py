@@ -1,0 +1,232 @@
+"""CPU emulation of the library's launch schedules (test helper).
+
+Runs the line passes that the planner emits (fftahip_debug_schedule /
+fftahip_debug_batch_schedule dry runs) in exact complex128 arithmetic with
+numpy, so that the pass geometry -- line/element addressing, multi-pass
+twiddle exponents, Bluestein chirp and spectrum indices, scratch layout --
+can be checked on the host against a reference FFT.  Every global access is
+bounds-checked and every pass is checked for two lines writing one address.
+
+`emulate_sweep` follows the kernel's in-LDS Stockham stage loop (per-thread
+butterfly coverage, stage index maps, stage twiddle indices) for one line.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_OFFTAB = 1, 2, 3, 4, 5, 6, 7, 8, 10
+EPT = 16
+STATIC_LDS = 5 * 8 * 256
+
+
+class ScheduleError(AssertionError):
+    pass
+
+
+def chirp_fwd(n: int) -> np.ndarray:
+    k = np.arange(n, dtype=np.int64)
+    p = (k * k) % (2 * n)
+    return np.exp(1j * np.pi * p / n)
+
+
+def bhat_of(n: int, m: int) -> np.ndarray:
+    b = chirp_fwd(n)
+    ext = np.zeros(m, dtype=np.complex128)
+    ext[:n] = b
+    ext[m - np.arange(1, n)] = b[1:]
+    return np.fft.fft(ext) / m
+
+
+def emulate_sweep(L: int, rad, Tl: int, rng=None) -> None:
+    """Replay the kernel's stage loop for one line; raise if any output slot is
+    missed or written twice, or if the result is not the DFT."""
+    rng = rng or np.random.default_rng(0)
+    x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
+    tw = np.exp(-2j * np.pi * np.arange(L) / L)
+    line = x.copy()
+    Ns = 1
+    for R in rad:
+        twst = L // (Ns * R)
+        Lr = L // R
+        NB = EPT // R
+        if NB < 1:
+            raise ScheduleError(f"radix {R} exceeds {EPT} registers")
+        if Tl * NB < Lr:
+            raise ScheduleError(f"L={L} radix {R}: {Tl} threads x {NB} butterflies < {Lr}")
+        new = np.zeros(L, dtype=np.complex128)
+        hits = np.zeros(L, dtype=np.int64)
+        for b in range(NB):
+            j = np.arange(Tl) + b * Tl
+            j = j[j < Lr]
+            if j.size == 0:
+                continue
+            t = np.arange(R)
+            v = line[j[:, None] + t[None, :] * Lr]
+            k = j % Ns
+            idx = t[None, :] * k[:, None] * twst
+            if idx.max(initial=0) >= L:
+                raise ScheduleError("stage twiddle index out of range")
+            v = v * tw[idx]
+            V = np.fft.fft(v, axis=1)
+            base = (j // Ns) * Ns * R + k
+            dst = base[:, None] + t[None, :] * Ns
+            if dst.max() >= L:
+                raise ScheduleError("stage write out of range")
+            new[dst] = V
+            np.add.at(hits, dst.ravel(), 1)
+        if not np.all(hits == 1):
+            raise ScheduleError(f"L={L} stage radix {R}: slots written {np.unique(hits)} times")
+        line = new
+        Ns *= R
+    if Ns != L:
+        raise ScheduleError(f"radices {rad} multiply to {Ns}, not {L}")
+    ref = np.fft.fft(x)
+    err = np.linalg.norm(line - ref) / max(np.linalg.norm(ref), 1e-300)
+    if err > 1e-12:
+        raise ScheduleError(f"sweep of L={L} radices {rad} wrong (rel err {err})")
+
+
+class Emulator:
+    def __init__(self, x: np.ndarray, out_elems: int, scratch_bytes: int, esize: int, offtab=None):
+        self.esize = esize
+        self.bufs = {
+            TK_IN: np.asarray(x, dtype=np.complex128).ravel(order="F").copy(),
+            TK_OUT: np.zeros(out_elems, dtype=np.complex128),
+            TK_SCRATCH: np.full(scratch_bytes // esize, np.nan + 1j * np.nan, dtype=np.complex128),
+        }
+        self.offtab = np.asarray(offtab if offtab is not None else [], dtype=np.int64)
+        self.sweeps_checked = set()
+
+    def _buf(self, tok):
+        kind, off = tok
+        if kind not in self.bufs:
+            raise ScheduleError(f"unexpected buffer token {tok}")
+        if off % self.esize:
+            raise ScheduleError("misaligned buffer offset")
+        return self.bufs[kind], off // self.esize
+
+    def run(self, launches, lds_per_block: int) -> None:
+        for la in launches:
+            if la["op"] == 1:
+                src, so = self._buf(la["in"])
+                dst, do = self._buf(la["out"])
+                n = la["bytes"] // self.esize
+                dst[do:do + n] = src[so:so + n]
+                continue
+            self._pass(la, lds_per_block)
+
+    def _pass(self, p, lds_per_block):
+        L, C, Tl = p["L"], p["C"], p["Tl"]
+        if C < 1 or C > 256 or C * Tl > p["block"]:
+            raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
+        if p["ldl"] < L or p["lds"] != C * p["ldl"] * self.esize:
+            raise ScheduleError("LDS size mismatch")
+        if p["lds"] + STATIC_LDS > lds_per_block:
+            raise ScheduleError(f"LDS {p['lds']} + static exceeds {lds_per_block}")
+        if p["grid"] * C < p["nlines"]:
+            raise ScheduleError("grid does not cover all lines")
+        if p["xcd_remap"] and p["grid"] % 8:
+            raise ScheduleError("XCD remap needs grid % 8 == 0")
+        key = (L, tuple(p["rad"]), Tl)
+        if key not in self.sweeps_checked:
+            emulate_sweep(L, p["rad"], Tl)
+            self.sweeps_checked.add(key)
+        if p["tw"] is None or p["tw"][0] != TK_STAGE or p["tw"][1] != L:
+            raise ScheduleError("stage twiddle table does not match L")
+        nl = p["nlines"]
+        n0, n1, n2 = p["n"]
+        line = np.arange(nl, dtype=np.int64)
+        idx = [line % n0]
+        r = line // n0
+        idx.append(r % n1)
+        r = r // n1
+        idx.append(r % n2)
+        idx.append(r // n2)
+        a, b, ca, cb = p["a"], p["b"], p["ca"], p["cb"]
+        oin = sum(idx[d] * a[d] for d in range(4))
+        oout = sum(idx[d] * b[d] for d in range(4))
+        if p["in_tab"] is not None:
+            base = p["in_tab"][1] // 8
+            oin = oin + self.offtab[base + idx[3]]
+        if p["out_tab"] is not None:
+            base = p["out_tab"][1] // 8
+            oout = oout + self.offtab[base + idx[3]]
+        cin = sum(idx[d] * ca[d] for d in range(4))
+        cout = sum(idx[d] * cb[d] for d in range(4))
+        t = np.arange(L, dtype=np.int64)
+        src, so = self._buf(p["in"])
+        dst, do = self._buf(p["out"])
+        addr_in = so + oin[:, None] + t[None, :] * p["ae"]
+        valid_in = np.ones_like(addr_in, dtype=bool)
+        chirp = bhat = None
+        if p["chirp"] is not None:
+            chirp = chirp_fwd(p["chirp"][1])
+        if p["bhat"] is not None:
+            M = p["bhat"][1]
+            bhat = bhat_of(p["chirp_n"], M)
+        if p["load_mode"] == 1:
+            n = cin[:, None] + t[None, :] * p["cae"]
+            valid_in = n < p["chirp_n"]
+        sel = addr_in[valid_in]
+        if sel.size and (sel.min() < 0 or sel.max() >= src.size):
+            raise ScheduleError(f"load out of bounds [{sel.min()}, {sel.max()}] of {src.size}")
+        v = np.zeros(addr_in.shape, dtype=np.complex128)
+        v[valid_in] = src[sel]
+        if np.isnan(v[valid_in]).any():
+            raise ScheduleError("pass reads scratch that no earlier pass wrote")
+        if p["conj_in"]:
+            v = np.conj(v)
+        if p["load_mode"] == 1:
+            n = cin[:, None] + t[None, :] * p["cae"]
+            nn = np.where(valid_in, n, 0)
+            v = np.where(valid_in, v * np.conj(chirp[nn]), 0)
+        if p["pretw"]:
+            M = p["tw_lo"][1]
+            m = idx[p["twdim"]][:, None] * t[None, :] * p["tw_scale"]
+            if m.max() >= M:
+                raise ScheduleError("pass twiddle exponent out of range")
+            v = v * np.exp(-2j * np.pi * m / M)
+        v = np.fft.fft(v, axis=1)
+        if p["mid_mode"] == 1:
+            if bhat is None or bhat.size != L:
+                raise ScheduleError("mid spectrum table does not match L")
+            v = np.conj(v * bhat[None, :])
+            v = np.fft.fft(v, axis=1)
+        addr_out = do + oout[:, None] + t[None, :] * p["be"]
+        valid_out = np.ones_like(addr_out, dtype=bool)
+        if p["store_mode"] == 1:
+            n = cout[:, None] + t[None, :] * p["cbe"]
+            if n.max() >= bhat.size:
+                raise ScheduleError("spectrum index out of range")
+            v = np.conj(v * bhat[n])
+        elif p["store_mode"] == 2:
+            n = cout[:, None] + t[None, :] * p["cbe"]
+            valid_out = n < p["chirp_n"]
+            nn = np.where(valid_out, n, 0)
+            v = np.conj(chirp[nn] * v)
+        if p["conj_out"]:
+            v = np.conj(v)
+        v = v * p["scale"]
+        sel = addr_out[valid_out]
+        if sel.size and (sel.min() < 0 or sel.max() >= dst.size):
+            raise ScheduleError(f"store out of bounds [{sel.min()}, {sel.max()}] of {dst.size}")
+        if np.unique(sel).size != sel.size:
+            raise ScheduleError("two lines store to one address")
+        dst[sel] = v[valid_out]
+
+
+def run_plan_schedule(sched: dict, x: np.ndarray, esize: int, lds_per_block: int) -> np.ndarray:
+    em = Emulator(x, x.size, sched["scratch_bytes"], esize)
+    em.run(sched["launches"], lds_per_block)
+    return em.bufs[TK_OUT].reshape(x.shape, order="F")
+
+
+def run_batch_schedule(sched: dict, x: np.ndarray, esize: int, lds_per_block: int) -> np.ndarray:
+    em = Emulator(x, x.size, sched["scratch_bytes"], esize, offtab=sched["offtab"])
+    ngroups = len(sched["groups"])
+    maxp = max((len(g) for g in sched["groups"]), default=0)
+    for ph in range(maxp):
+        for g in range(ngroups):
+            if ph < len(sched["groups"][g]):
+                em.run([sched["groups"][g][ph]], lds_per_block)
+    return em.bufs[TK_OUT]
