@@ -77,8 +77,9 @@ struct LinePass {
   int32_t ldl;  // LDS row pitch in elements (>= L)
   int32_t col_in, col_out;  // 1: consecutive lines are adjacent in memory (coalesce across lines)
   FastDiv divC, divL;       // by C and by L
-  int64_t nlines;
+  int64_t nlines;           // < 2^31
   int64_t n[3];
+  FastDiv divn[3];          // by n[0], n[1], n[2]
   int64_t a[4], b[4];
   int64_t ae, be;
   const int64_t* in_tab;

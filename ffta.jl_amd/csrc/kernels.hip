@@ -110,6 +110,10 @@ struct LineShared {
 };
 static_assert(sizeof(LineShared) == kLineSharedBytes, "static LDS size");
 
+// Global accesses are issued in batches of kU per thread so that each thread
+// keeps kU loads in flight before it consumes any of them.
+constexpr int kU = 8;
+
 template <class Rt, int EPT, int BLOCK>
 __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, cx<Rt>* lds, LineShared& sh) {
   const int tid = threadIdx.x;
@@ -121,14 +125,15 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
   const int nvalid = rem < C ? (int)rem : C;
 
   for (int c = tid; c < nvalid; c += BLOCK) {
-    const int64_t line = line0 + c;
+    const uint32_t line = (uint32_t)(line0 + c);
     int64_t idx[4];
-    idx[0] = line % p.n[0];
-    int64_t r = line / p.n[0];
-    idx[1] = r % p.n[1];
-    r /= p.n[1];
-    idx[2] = r % p.n[2];
-    idx[3] = r / p.n[2];
+    uint32_t q = fdiv(line, p.divn[0]);
+    idx[0] = line - q * p.divn[0].d;
+    uint32_t q1 = fdiv(q, p.divn[1]);
+    idx[1] = q - q1 * p.divn[1].d;
+    uint32_t q2 = fdiv(q1, p.divn[2]);
+    idx[2] = q1 - q2 * p.divn[2].d;
+    idx[3] = q2;
     int64_t oi = idx[0] * p.a[0] + idx[1] * p.a[1] + idx[2] * p.a[2] + idx[3] * p.a[3];
     int64_t oo = idx[0] * p.b[0] + idx[1] * p.b[1] + idx[2] * p.b[2] + idx[3] * p.b[3];
     if (p.in_tab) oi += p.in_tab[idx[3]];
@@ -147,40 +152,65 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
   const cx<Rt>* __restrict__ chirp = reinterpret_cast<const cx<Rt>*>(p.chirp);
   const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
   const int total = C * L;
+  const bool chirp_in = p.load_mode == LOAD_CHIRP;
+  const int64_t twmask = (1ll << p.tw_sh) - 1;
 
   // ---- load (global -> LDS), coalesced across lines or along a line
-  for (int e = tid; e < total; e += BLOCK) {
-    int c, t;
-    if (p.col_in) {
-      t = (int)fdiv((uint32_t)e, p.divC);
-      c = e - t * C;
-    } else {
-      c = (int)fdiv((uint32_t)e, p.divL);
-      t = e - c * L;
-    }
-    if (c >= nvalid) continue;
-    cx<Rt> v;
-    if (p.load_mode == LOAD_CHIRP) {
-      const int64_t n = sh.cin[c] + (int64_t)t * p.cae;
-      if (n < p.chirp_n) {
-        v = in[sh.in[c] + (int64_t)t * p.ae];
-        if (p.conj_in) v = cconj(v);
-        v = cmul(v, cconj(chirp[n]));
-      } else {
-        v = {(Rt)0, (Rt)0};
+  for (int e0 = 0; e0 < total; e0 += BLOCK * kU) {
+    cx<Rt> v[kU];
+    cx<Rt> ch[kU];
+    double2 wlo[kU], whi[kU];
+    int slot[kU];
+    bool live[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int e = e0 + u * BLOCK + tid;
+      slot[u] = -1;
+      live[u] = false;
+      v[u] = {(Rt)0, (Rt)0};
+      if (e < total) {
+        int c, t;
+        if (p.col_in) {
+          t = (int)fdiv((uint32_t)e, p.divC);
+          c = e - t * C;
+        } else {
+          c = (int)fdiv((uint32_t)e, p.divL);
+          t = e - c * L;
+        }
+        if (c < nvalid) {
+          slot[u] = c * ldl + t;
+          live[u] = true;
+          if (chirp_in) {
+            const int64_t n = sh.cin[c] + (int64_t)t * p.cae;
+            live[u] = n < p.chirp_n;
+            if (live[u]) ch[u] = chirp[n];
+          }
+          if (live[u]) {
+            v[u] = in[sh.in[c] + (int64_t)t * p.ae];
+            if (p.pretw) {
+              const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
+              wlo[u] = p.tw_lo[m & twmask];
+              whi[u] = p.tw_hi[m >> p.tw_sh];
+            }
+          }
+        }
       }
-    } else {
-      v = in[sh.in[c] + (int64_t)t * p.ae];
-      if (p.conj_in) v = cconj(v);
     }
-    if (p.pretw) {
-      const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
-      const double2 lo = p.tw_lo[m & ((1ll << p.tw_sh) - 1)];
-      const double2 hi = p.tw_hi[m >> p.tw_sh];
-      const double2 w = make_double2(lo.x * hi.x - lo.y * hi.y, lo.x * hi.y + lo.y * hi.x);
-      v = cmul(v, from_d2<Rt>(w));
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (slot[u] < 0) continue;
+      cx<Rt> x = v[u];
+      if (live[u]) {
+        if (p.conj_in) x = cconj(x);
+        if (chirp_in) x = cmul(x, cconj(ch[u]));
+        if (p.pretw) {
+          const double2 w = make_double2(wlo[u].x * whi[u].x - wlo[u].y * whi[u].y,
+                                         wlo[u].x * whi[u].y + wlo[u].y * whi[u].x);
+          x = cmul(x, from_d2<Rt>(w));
+        }
+      }
+      lds[slot[u]] = x;
     }
-    lds[c * ldl + t] = v;
   }
   __syncthreads();
 
@@ -192,10 +222,25 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
   sweep<Rt, EPT>(myline, jl, active, p, tw);
 
   if (p.mid_mode == MID_BHAT) {
-    for (int e = tid; e < total; e += BLOCK) {
-      const int c = (int)fdiv((uint32_t)e, p.divL);
-      const int t = e - c * L;
-      if (c < nvalid) lds[c * ldl + t] = cconj(cmul(lds[c * ldl + t], bhat[t]));
+    for (int e0 = 0; e0 < total; e0 += BLOCK * kU) {
+      cx<Rt> f[kU];
+      int slot[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int e = e0 + u * BLOCK + tid;
+        slot[u] = -1;
+        if (e < total) {
+          const int c = (int)fdiv((uint32_t)e, p.divL);
+          const int t = e - c * L;
+          if (c < nvalid) {
+            slot[u] = c * ldl + t;
+            f[u] = bhat[t];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (slot[u] >= 0) lds[slot[u]] = cconj(cmul(lds[slot[u]], f[u]));
     }
     __syncthreads();
     sweep<Rt, EPT>(myline, jl, active, p, tw);
@@ -203,28 +248,52 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
 
   // ---- store (LDS -> global)
   const Rt scale = (Rt)p.scale;
-  for (int e = tid; e < total; e += BLOCK) {
-    int c, t;
-    if (p.col_out) {
-      t = (int)fdiv((uint32_t)e, p.divC);
-      c = e - t * C;
-    } else {
-      c = (int)fdiv((uint32_t)e, p.divL);
-      t = e - c * L;
+  const bool has_scale = p.scale != 1.0;
+  for (int e0 = 0; e0 < total; e0 += BLOCK * kU) {
+    cx<Rt> f[kU];
+    int slot[kU];
+    int64_t dst[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int e = e0 + u * BLOCK + tid;
+      slot[u] = -1;
+      if (e < total) {
+        int c, t;
+        if (p.col_out) {
+          t = (int)fdiv((uint32_t)e, p.divC);
+          c = e - t * C;
+        } else {
+          c = (int)fdiv((uint32_t)e, p.divL);
+          t = e - c * L;
+        }
+        if (c < nvalid) {
+          bool keep = true;
+          if (p.store_mode != STORE_PLAIN) {
+            const int64_t n = sh.cout[c] + (int64_t)t * p.cbe;
+            if (p.store_mode == STORE_BHAT) {
+              f[u] = bhat[n];
+            } else {
+              keep = n < p.chirp_n;
+              if (keep) f[u] = chirp[n];
+            }
+          }
+          if (keep) {
+            slot[u] = c * ldl + t;
+            dst[u] = sh.out[c] + (int64_t)t * p.be;
+          }
+        }
+      }
     }
-    if (c >= nvalid) continue;
-    cx<Rt> v = lds[c * ldl + t];
-    if (p.store_mode == STORE_BHAT) {
-      const int64_t n = sh.cout[c] + (int64_t)t * p.cbe;
-      v = cconj(cmul(v, bhat[n]));
-    } else if (p.store_mode == STORE_CHIRP) {
-      const int64_t n = sh.cout[c] + (int64_t)t * p.cbe;
-      if (n >= p.chirp_n) continue;
-      v = cconj(cmul(chirp[n], v));
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (slot[u] < 0) continue;
+      cx<Rt> x = lds[slot[u]];
+      if (p.store_mode == STORE_BHAT) x = cconj(cmul(x, f[u]));
+      else if (p.store_mode == STORE_CHIRP) x = cconj(cmul(f[u], x));
+      if (p.conj_out) x = cconj(x);
+      if (has_scale) x = cscale(x, scale);
+      out[dst[u]] = x;
     }
-    if (p.conj_out) v = cconj(v);
-    if (p.scale != 1.0) v = cscale(v, scale);
-    out[sh.out[c] + (int64_t)t * p.be] = v;
   }
 }
 
@@ -341,8 +410,10 @@ void allow_dynamic_lds(size_t bytes) {
 void init_kernels(size_t lds_per_block) {
   const size_t dyn = lds_per_block > (size_t)kLineSharedBytes ? lds_per_block - kLineSharedBytes : 0;
   allow_dynamic_lds<float, 16, 256>(dyn);
+  allow_dynamic_lds<float, 16, 512>(dyn);
   allow_dynamic_lds<float, 16, 1024>(dyn);
   allow_dynamic_lds<double, 16, 256>(dyn);
+  allow_dynamic_lds<double, 16, 512>(dyn);
   allow_dynamic_lds<double, 16, 1024>(dyn);
 }
 
@@ -350,9 +421,11 @@ hipError_t launch_line_pass(const LinePass& p, int precision, int block, size_t 
                             hipStream_t stream) {
   if (precision == 32) {
     if (block == 256) return launch_line_pass_t<float, 16, 256>(p, lds_bytes, grid, stream);
+    if (block == 512) return launch_line_pass_t<float, 16, 512>(p, lds_bytes, grid, stream);
     if (block == 1024) return launch_line_pass_t<float, 16, 1024>(p, lds_bytes, grid, stream);
   } else {
     if (block == 256) return launch_line_pass_t<double, 16, 256>(p, lds_bytes, grid, stream);
+    if (block == 512) return launch_line_pass_t<double, 16, 512>(p, lds_bytes, grid, stream);
     if (block == 1024) return launch_line_pass_t<double, 16, 1024>(p, lds_bytes, grid, stream);
   }
   return hipErrorInvalidValue;
@@ -362,11 +435,14 @@ hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_t
                                    int precision, int block, size_t lds_bytes, int64_t grid, hipStream_t stream) {
   if (precision == 32) {
     if (block == 256) return launch_group_t<float, 16, 256>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
+    if (block == 512) return launch_group_t<float, 16, 512>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
     if (block == 1024)
       return launch_group_t<float, 16, 1024>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
   } else {
     if (block == 256)
       return launch_group_t<double, 16, 256>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
+    if (block == 512)
+      return launch_group_t<double, 16, 512>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
     if (block == 1024)
       return launch_group_t<double, 16, 1024>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
   }

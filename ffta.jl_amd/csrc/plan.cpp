@@ -244,7 +244,7 @@ static bool make_inner(int64_t L, Inner& in) {
   if (Tl > 1024) return false;
   in.rad = rad;
   in.Tl = (int)Tl;
-  in.block = Tl <= 256 ? 256 : 1024;
+  in.block = Tl <= 256 ? 256 : (Tl <= 512 ? 512 : 1024);
   return true;
 }
 
@@ -356,13 +356,18 @@ static int64_t lds_budget() {
   lds -= kLineSharedBytes + 1024;  // the kernel's static per-line tables
   return cap > 0 ? std::min(cap, lds) : lds;
 }
-static int64_t min_lines(int prec) { return prec == 32 ? 8 : 4; }
+// Adjacent lines per workgroup for strided passes: 128-byte segments.
+static int64_t min_lines(int prec) { return prec == 32 ? 16 : 8; }
+// LDS per workgroup for strided (multi-pass) tiles; small enough for
+// two or more resident workgroups per CU.
+static int64_t lds_budget();
+static int64_t pass_budget() { return std::min<int64_t>(env_i64("FFTAHIP_PASS_LDS", 65536), lds_budget()); }
 static int64_t single_max(int prec) {
   int64_t m = lds_budget() / (int64_t)csize(prec);
   return std::min<int64_t>(m, 1024 * kEPT);
 }
 static int64_t pass_max(int prec) {
-  const int64_t dflt = std::min<int64_t>(1024, lds_budget() / ((int64_t)csize(prec) * min_lines(prec)));
+  const int64_t dflt = std::min<int64_t>(1024, pass_budget() / ((int64_t)csize(prec) * min_lines(prec)));
   const int64_t v = env_i64("FFTAHIP_PASS_MAX", dflt);
   return std::max<int64_t>(v, 16);
 }
@@ -486,7 +491,10 @@ static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
   int64_t total = 1;
   for (int i = 0; i < 4; ++i) total *= slots[i].n;
   p.nlines = total;
-  for (int i = 0; i < 3; ++i) p.n[i] = slots[i].n;
+  for (int i = 0; i < 3; ++i) {
+    p.n[i] = slots[i].n;
+    p.divn[i] = make_fastdiv((uint32_t)slots[i].n);
+  }
   p.twdim = 0;
   for (int i = 0; i < 4; ++i) {
     p.a[i] = slots[i].a;
@@ -518,17 +526,24 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   }
   p.Tl = in.Tl;
   const int64_t esz = (int64_t)csize(prec);
-  int64_t C = std::min<int64_t>(kMaxLines, std::max<int64_t>(1, in.block / in.Tl));
-  const int64_t budget = lds_budget();
-  int64_t ldl = in.L + ((p.col_in || p.col_out) ? 1 : 0);
+  const bool strided = p.col_in || p.col_out;
+  // lines per workgroup: fill 256 threads; strided tiles also want
+  // min_lines adjacent lines (128-byte segments) within the pass budget
+  int64_t C = std::max<int64_t>(1, 256 / in.Tl);
+  if (strided) C = std::max<int64_t>(C, min_lines(prec));
+  C = std::min<int64_t>(C, kMaxLines);
+  C = std::min<int64_t>(C, std::max<int64_t>(1, 1024 / in.Tl));
+  int64_t ldl = in.L + (strided ? 1 : 0);
+  const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pass_budget(), ldl * esz)) : lds_budget();
   while (C > 1 && C * ldl * esz > budget) --C;
   if (C > p.nlines) C = std::max<int64_t>(1, p.nlines);
   if (C == 1) ldl = in.L;
+  const int64_t threads = C * in.Tl;
   p.C = (int32_t)C;
   p.ldl = (int32_t)ldl;
   p.divC = make_fastdiv((uint32_t)C);
   p.divL = make_fastdiv((uint32_t)in.L);
-  L.block = in.block;
+  L.block = threads <= 256 ? 256 : (threads <= 512 ? 512 : 1024);
   L.lds = (size_t)(C * ldl * esz);
   L.grid = (p.nlines + C - 1) / C;
   p.xcd_remap = (L.grid >= 16 && L.grid % 8 == 0) ? 1 : 0;
@@ -740,8 +755,15 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
   ffta_nodes(lp.ffta, N, cutoff, 1, 1);
   lp.blue = !smooth13(N);
   if (lp.blue) {
-    int64_t M = 1;
-    while (M < 2 * N - 1) M <<= 1;
+    // smallest 5-smooth length >= 2N-1 (FFTA pads to a power of two,
+    // src/algos.jl:345; any length >= 2N-1 gives the same convolution)
+    int64_t M = 2 * N - 1;
+    for (;; ++M) {
+      int64_t m = M;
+      for (int q : {2, 3, 5})
+        while (m % q == 0) m /= q;
+      if (m == 1) break;
+    }
     lp.M = M;
   } else {
     lp.M = N;
@@ -1253,31 +1275,40 @@ static fftahip_status batch_build(fftahip_batch b, const void* d_in, void* d_out
     per_group.push_back(L);
   }
   // phase p = the p-th launch of every group; within a phase, one grouped
-  // launch per block size.
+  // launch per (block size, LDS class) so that small lines are not held to
+  // the occupancy of the largest LDS footprint.
+  static const size_t kLdsClass[] = {16384, 32768, 65536, 98304, 163840};
+  auto lds_class = [](size_t lds) {
+    for (size_t c : kLdsClass)
+      if (lds <= c) return c;
+    return (size_t)163840;
+  };
   size_t maxp = 0;
   for (auto& L : per_group) maxp = std::max(maxp, L.size());
   for (size_t ph = 0; ph < maxp; ++ph) {
-    for (int block : {256, 1024}) {
-      fftahip_batch_s::Phase P;
-      P.block = block;
-      P.lds = 0;
-      P.ngroups = 0;
-      P.pass_off = b->host_passes.size();
-      P.tile_off = b->host_tiles.size();
-      uint32_t tiles = 0;
-      for (auto& L : per_group) {
-        if (ph >= L.size() || L[ph].block != block || L[ph].grid <= 0) continue;
-        LinePass lp = L[ph].lp;
-        lp.xcd_remap = 0;
-        b->host_passes.push_back(lp);
-        b->host_tiles.push_back(tiles);
-        tiles += (uint32_t)L[ph].grid;
-        P.lds = std::max(P.lds, L[ph].lds);
-        ++P.ngroups;
+    for (int block : {256, 512, 1024}) {
+      for (size_t cls : kLdsClass) {
+        fftahip_batch_s::Phase P;
+        P.block = block;
+        P.lds = 0;
+        P.ngroups = 0;
+        P.pass_off = b->host_passes.size();
+        P.tile_off = b->host_tiles.size();
+        uint32_t tiles = 0;
+        for (auto& L : per_group) {
+          if (ph >= L.size() || L[ph].block != block || L[ph].grid <= 0 || lds_class(L[ph].lds) != cls) continue;
+          LinePass lp = L[ph].lp;
+          lp.xcd_remap = 0;
+          b->host_passes.push_back(lp);
+          b->host_tiles.push_back(tiles);
+          tiles += (uint32_t)L[ph].grid;
+          P.lds = std::max(P.lds, L[ph].lds);
+          ++P.ngroups;
+        }
+        if (P.ngroups == 0) continue;
+        P.grid = tiles;
+        b->phases.push_back(P);
       }
-      if (P.ngroups == 0) continue;
-      P.grid = tiles;
-      b->phases.push_back(P);
     }
   }
   if (g_dry) {

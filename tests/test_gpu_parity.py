@@ -1,0 +1,112 @@
+"""GPU parity of the HIP transform path against the oracle (FFTA's algorithm on the CPU).
+
+Tolerances (relative L2 error against the FP64 oracle on the same inputs):
+  ComplexF64: <= 1e-14 * log2 N;  ComplexF32: <= 2e-6 * log2 N;  N = 1 exact.
+For a CF32 result the oracle runs in FP64 on the CF32 inputs (the reference's
+Float64 path is the bar, as the reference's own accuracy test does,
+test/onedim/accuracy.jl:48-51).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import tol
+from gpu_util import empty_device, to_device, to_host
+
+pytestmark = pytest.mark.gpu
+
+SIZES_1D = [1, 2, 3, 4, 5, 7, 8, 9, 11, 12, 13, 15, 16, 17, 27, 32, 49, 64, 73, 100, 103, 121, 128, 169, 243, 256,
+            275, 729, 1000, 1024, 2187, 3000, 4096, 5329, 8192, 10007, 16384, 65536, 177147, 262144]
+
+
+def _rand(n_or_shape, dtype, seed):
+    rng = np.random.default_rng(seed)
+    shape = (n_or_shape,) if isinstance(n_or_shape, int) else tuple(n_or_shape)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    return np.asfortranarray(x.astype(dtype))
+
+
+def _gpu_transform(torch, x, direction, region=None):
+    from fftahip.plan import Plan, mul_
+
+    prec = 32 if x.dtype == np.complex64 else 64
+    p = Plan(x.shape, prec, region, direction)
+    tx, dx = to_device(torch, x)
+    ty, dy = empty_device(torch, x.shape, x.dtype)
+    mul_(dy, p, dx)
+    return to_host(torch, ty, x.shape)
+
+
+def _oracle(x, direction, region=None):
+    x64 = np.asfortranarray(x.astype(np.complex128))
+    if x.ndim == 1 and region is None:
+        return oracle.fft(x64, direction)
+    return oracle.fftn(x64, region, direction)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("direction", [-1, 1])
+@pytest.mark.parametrize("n", SIZES_1D)
+def test_1d_vs_oracle(torch_cuda, n, direction, precision):
+    dt = np.complex64 if precision == 32 else np.complex128
+    x = _rand(n, dt, seed=n * 7 + (direction > 0))
+    print(f"case 1d n={n} dir={direction} prec={precision}", flush=True)
+    y = _gpu_transform(torch_cuda, x, direction)
+    ref = _oracle(x, direction)
+    err = oracle.relerr(y, ref)
+    assert err <= tol(n, precision), f"n={n} dir={direction} CF{precision}: relerr {err:.3e} > {tol(n, precision):.3e}"
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("shape", [(8, 11), (11, 8), (64, 64), (65, 64), (100, 103), (27, 128), (256, 243),
+                                   (1, 5), (5, 1), (300, 218)])
+def test_2d_vs_oracle(torch_cuda, shape, precision):
+    dt = np.complex64 if precision == 32 else np.complex128
+    for direction in (-1, 1):
+        x = _rand(shape, dt, seed=sum(shape) + direction)
+        print(f"case 2d shape={shape} dir={direction} prec={precision}", flush=True)
+        y = _gpu_transform(torch_cuda, x, direction)
+        ref = _oracle(x, direction)
+        n = int(np.prod(shape))
+        err = oracle.relerr(y, ref)
+        assert err <= tol(n, precision), f"{shape} dir={direction}: relerr {err:.3e}"
+
+
+@pytest.mark.parametrize("shape,region", [((5, 6, 7), 1), ((5, 6, 7), 2), ((5, 6, 7), 3), ((5, 6, 7), (1, 3)),
+                                          ((5, 6, 7), (2, 3)), ((4, 14, 9), None), ((103, 5, 13), None),
+                                          ((26, 33, 35, 4), (2, 4)), ((3, 3, 3, 3, 3), None)])
+def test_nd_regions_vs_oracle(torch_cuda, shape, region):
+    x = _rand(shape, np.complex128, seed=len(shape))
+    for direction in (-1, 1):
+        print(f"case nd shape={shape} region={region} dir={direction}", flush=True)
+        y = _gpu_transform(torch_cuda, x, direction, region)
+        ref = _oracle(x, direction, region)
+        reg = (region,) if isinstance(region, int) else (region or tuple(range(1, len(shape) + 1)))
+        n = int(np.prod([shape[r - 1] for r in reg]))
+        assert oracle.relerr(y, ref) <= tol(n, 64)
+
+
+def test_ones_delta_closed_form(torch_cuda):
+    # test/onedim/complex_forward.jl:3-12 and complex_backward.jl:4-11
+    for n in [8, 11, 15, 16, 27, 100, 103, 275]:
+        for direction in (-1, 1):
+            x = np.ones(n, dtype=np.complex128)
+            print(f"case ones n={n} dir={direction}", flush=True)
+            y = _gpu_transform(torch_cuda, x, direction)
+            ref = np.zeros(n, dtype=np.complex128)
+            ref[0] = n
+            assert np.max(np.abs(y - ref)) <= 1e-12
+
+
+def test_ndim_333_table(torch_cuda):
+    # test/ndim/minimal_complex.jl:9-26 (delta at (2,2,2), 1-based)
+    import os
+
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ndim_333.npz"))["y_ref"]
+    y = np.zeros((3, 3, 3), dtype=np.complex128, order="F")
+    y[1, 1, 1] = 1
+    print("case ndim_333", flush=True)
+    out = _gpu_transform(torch_cuda, y, -1)
+    assert np.max(np.abs(out - ref)) <= 1e-14

@@ -1,0 +1,201 @@
+"""Host-side tests of the library (no GPU): C ABI exports, planner, the error
+contract of src/plan.jl / test/argument_checking.jl, and CPU emulation of the
+launch schedules the planner emits."""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+import emulator as E
+import fftahip
+from fftahip import _lib
+from fftahip import plan as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "fftahip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fftahip_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib.load()
+    names = _header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/fftahip.h but not exported"
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of sync with the header"
+
+
+def test_version_and_status_strings():
+    assert "fftahip" in fftahip.version()
+    L = _lib.load()
+    assert L.fftahip_status_string(1).decode() == "DimensionMismatch"
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 7, 9, 15, 16, 17, 243, 255, 256, 1000, 3000, 8192, 65535, 131072,
+                               1000003, 2**20 + 7, 31104000])
+def test_fastdiv_exact(d):
+    rng = np.random.default_rng(d)
+    n = np.concatenate([np.arange(0, 70000, dtype=np.uint64),
+                        rng.integers(0, 2**31, 200000, dtype=np.uint64),
+                        np.array([2**31 - 1, 2**32 - 1, d - 1, d, d + 1, 2 * d - 1, 2 * d], dtype=np.uint64)])
+    n = n[n < 2**32].astype(np.uint32)
+    q = P.debug_fastdiv(d, n)
+    assert np.array_equal(q.astype(np.uint64), n.astype(np.uint64) // d)
+
+
+# ---------------------------------------------------------------- plan API / error contract
+
+
+def test_plan_size_and_domain_error():
+    # test/argument_checking.jl:229-237 (plan_fft sizes; non-Bluestein lengths plan on the host)
+    for sz in [(8, 11), (6, 24, 25), (100,)]:
+        x = np.zeros(sz, dtype=np.complex128)
+        p = P.plan_fft(x)
+        assert p.size() == sz
+        assert p.size(1 + len(sz)) == 1
+        with pytest.raises(P.DomainError):
+            p.size(0)
+        with pytest.raises(P.DomainError):
+            p.size(-1)
+
+
+def test_empty_array_dimension_mismatch():
+    # test/onedim/complex_forward.jl:35-37, twodim/complex_forward.jl:45-47
+    with pytest.raises(P.DimensionMismatch, match="strictly positive"):
+        P.plan_fft(np.zeros(0, dtype=np.complex128))
+    with pytest.raises(P.DimensionMismatch):
+        P.plan_bfft(np.zeros((0, 0), dtype=np.complex128))
+
+
+def test_mismatch_plan_and_array():
+    # test/argument_checking.jl:131-188 (complex parts)
+    xc1 = np.zeros(3, dtype=np.complex128)
+    xc1p = np.zeros(4, dtype=np.complex128)
+    with pytest.raises(P.DimensionMismatch):
+        P.plan_fft(xc1) * xc1p
+    with pytest.raises(P.DimensionMismatch):
+        P.plan_bfft(xc1) * xc1p
+    xc2 = np.zeros((3, 3), dtype=np.complex128)
+    xc2p = np.zeros((4, 4), dtype=np.complex128)
+    for region in (1, 2):
+        with pytest.raises(P.DimensionMismatch):
+            P.plan_fft(xc2, region) * xc2p
+        with pytest.raises(P.DimensionMismatch):
+            P.plan_bfft(xc2, region) * xc2p
+    with pytest.raises(P.DimensionMismatch):
+        P.plan_fft(xc2) * xc2p
+
+
+def test_region_outside_dimensions():
+    # test/argument_checking.jl:189-203
+    xc3 = np.zeros((3, 3, 3), dtype=np.complex128)
+    yc3 = np.zeros((5, 5, 5), dtype=np.complex128)
+    with pytest.raises(P.DimensionMismatch):
+        P.plan_fft(xc3) * yc3
+    invalid_p = P.plan_fft(np.zeros((3,) * 5, dtype=np.complex128), range(3, 6))
+    with pytest.raises(P.DimensionMismatch, match=r"^Plan region is outside array dimensions\.$"):
+        invalid_p * xc3
+    xc4 = np.zeros((1, 5, 5, 5), dtype=np.complex128, order="F")
+    with pytest.raises(P.DimensionMismatch, match=r"^Plan region is outside array dimensions\.$"):
+        P.mul_(xc4, invalid_p, xc4)
+
+
+def test_mismatch_input_output():
+    # test/argument_checking.jl:206-227
+    x1 = np.zeros(3, dtype=np.complex128)
+    y1 = np.zeros(4, dtype=np.complex128)
+    with pytest.raises(P.DimensionMismatch):
+        P.mul_(y1, P.plan_fft(x1), x1)
+    for N in (2, 3, 4):
+        xN = np.zeros((3,) * N, dtype=np.complex128, order="F")
+        yN = np.zeros((4,) * N, dtype=np.complex128, order="F")
+        for region in range(1, N + 1):
+            with pytest.raises(P.DimensionMismatch):
+                P.mul_(yN, P.plan_fft(xN, region), xN)
+        with pytest.raises(P.DimensionMismatch):
+            P.mul_(yN, P.plan_fft(xN), xN)
+        with pytest.raises(P.DimensionMismatch):
+            P.mul_(yN, P.plan_fft(xN, range(1, N)), xN)
+
+
+def test_mutated_region_is_invalid():
+    # test/argument_checking.jl:249-266
+    for n in (3, 4, 5):
+        x = np.zeros((2,) * n, dtype=np.complex128, order="F")
+        p1 = P.plan_fft(x, list(range(1, n)))
+        p1.region.append(n)
+        with pytest.raises(P.DimensionMismatch, match=r"^Region is invalid\.$"):
+            p1 * x
+        p2 = P.plan_fft(x, list(range(1, n)))
+        p2.region[0:2] = [2, 1]
+        with pytest.raises(P.DimensionMismatch, match=r"^Region is invalid\.$"):
+            p2 * x
+
+
+def test_describe_reports_ffta_tree():
+    p = P.plan_fft(np.zeros(275, dtype=np.complex128))
+    d = p.describe()
+    tree = [(q["type"], q["sz"], q["s_in"], q["s_out"]) for q in d["dims"][0]["ffta"]]
+    assert tree == [("COMPOSITE_FFT", 275, 1, 1), ("COMPOSITE_FFT", 25, 11, 11), ("DFT", 5, 5, 55),
+                    ("DFT", 5, 55, 1), ("DFT", 11, 25, 1)]
+
+
+# ---------------------------------------------------------------- schedule emulation
+
+SHAPES_1D = [1, 2, 3, 16, 17, 64, 100, 103, 243, 1000, 1009, 2187, 3000, 4096, 5329, 8192, 10007, 16384,
+             30000, 65536, 131072, 177147, 262144]
+
+
+def _check_schedule(shape, prec, direction, region=None, lds=160 * 1024):
+    s = P.debug_schedule(shape, prec, direction, region, lds)
+    rng = np.random.default_rng(sum(shape))
+    x = np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    y = E.run_plan_schedule(s, x, 8 if prec == 32 else 16, lds)
+    reg = P._region_tuple(region, len(shape))
+    axes = [r - 1 for r in reg if r <= len(shape)]
+    if direction == -1:
+        ref = np.fft.fftn(x, axes=axes)
+    else:
+        ref = np.fft.ifftn(x, axes=axes) * np.prod([shape[a] for a in axes])
+    err = np.linalg.norm(y - ref) / max(np.linalg.norm(ref), 1e-300)
+    assert err < 1e-12, (shape, prec, direction, region, lds, err)
+
+
+@pytest.mark.parametrize("lds", [160 * 1024, 64 * 1024])
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("n", SHAPES_1D)
+def test_schedule_1d(n, prec, lds):
+    _check_schedule((n,), prec, -1 if n % 2 else 1, None, lds)
+
+
+@pytest.mark.parametrize("shape,region", [((8, 11), None), ((64, 64), None), ((30, 21), 2), ((5, 6, 7), 2),
+                                          ((5, 6, 7), (1, 3)), ((5, 6, 7), None), ((4096, 8), 2),
+                                          ((3000, 9), None), ((103, 5, 13), None), ((1024, 1024), None),
+                                          ((17, 4099), None), ((26, 33, 35, 4), (2, 4)), ((3, 5), 3)])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_schedule_nd(shape, region, prec):
+    _check_schedule(shape, prec, -1, region)
+    _check_schedule(shape, prec, 1, region, 64 * 1024)
+
+
+def test_schedule_batch_uneven():
+    rng = np.random.default_rng(5)
+    lengths = np.concatenate([rng.integers(1, 300, 40), [4096, 4099, 8192, 16384, 20011, 30000, 65536, 65521, 59049],
+                              rng.integers(16, 70000, 12)])
+    dirs = np.where(rng.uniform(size=lengths.size) < 0.5, -1, 1)
+    s = P.debug_batch_schedule(lengths, dirs, 32)
+    x = rng.standard_normal(lengths.sum()) + 1j * rng.standard_normal(lengths.sum())
+    y = E.run_batch_schedule(s, x, 8, 160 * 1024)
+    off = 0
+    for n, d in zip(lengths, dirs):
+        xi = x[off:off + n]
+        ref = np.fft.fft(xi) if d == -1 else np.fft.ifft(xi) * n
+        assert np.linalg.norm(y[off:off + n] - ref) <= 1e-12 * max(np.linalg.norm(ref), 1e-300)
+        off += n
