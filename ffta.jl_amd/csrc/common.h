@@ -7,8 +7,8 @@ namespace fftahip {
 
 constexpr int kMaxStages = 12;      // radix stages inside one in-LDS line FFT
 constexpr int kMaxLines = 256;      // lines per workgroup, at most
-// static LDS of a line-pass workgroup: five int64 per line
-constexpr int kLineSharedBytes = 5 * 8 * kMaxLines;
+// static LDS of a line-pass workgroup: six int64 per line
+constexpr int kLineSharedBytes = 6 * 8 * kMaxLines;
 
 // Unsigned 32-bit division by a runtime constant: q = (umulhi(n, m) + n) >> s.
 // Exact for every n < 2^32 when d < 2^31 (round-up method of Granlund and
@@ -46,7 +46,7 @@ enum LoadMode : int32_t {
 // What a line pass does between its first and second radix sweep.
 enum MidMode : int32_t {
   MID_NONE = 0,   // one sweep only
-  MID_BHAT = 1,   // Bluestein in one pass: v = conj(v * bhat[k]), then sweep again
+  MID_BHAT = 1,   // Bluestein spectrum product: v = conj(v * bhat[n]), then sweep again
 };
 
 // What a line pass does when it stores a line.
@@ -61,12 +61,16 @@ enum StoreMode : int32_t {
 // optional fused pre/post operations.  Lines are indexed (i0, i1, i2, i3)
 // with i0 fastest (extents n[0..2], i3 takes the rest); element t of a line
 // is read from
-//   in  + sum_d i_d*a[d] + (in_tab ? in_tab[i3] : 0) + t*ae
+//   buf[in_buf] + in_off + sum_d i_d*a[d] + (in_tab ? in_tab[i3] : 0) + t*ae
 // and output element k is written to
-//   out + sum_d i_d*b[d] + (out_tab ? out_tab[i3] : 0) + k*be.
+//   buf[out_buf] + out_off + sum_d i_d*b[d] + (out_tab ? out_tab[i3] : 0) + k*be.
+// Buffers a pass reads or writes, chosen at launch time (so that a schedule
+// uploaded once can run on any arrays).
+enum BufId : int32_t { BUF_IN = 0, BUF_OUT = 1, BUF_SCRATCH = 2 };
+
 struct LinePass {
-  const void* in;
-  void* out;
+  int32_t in_buf, out_buf;   // BufId
+  int64_t in_off, out_off;   // element offsets into those buffers
   int32_t L;
   int32_t nst;
   int32_t rad[kMaxStages];
@@ -74,7 +78,8 @@ struct LinePass {
   FastDiv nsdiv[kMaxStages];      // by Ns (product of earlier radices) per stage
   int32_t C;    // lines per workgroup
   int32_t Tl;   // threads per line (C*Tl <= block, C <= kMaxLines)
-  int32_t ldl;  // LDS row pitch in elements (>= L)
+  int32_t ldl;  // LDS row pitch in elements (>= padded L)
+  int32_t lds_sh;  // LDS padding: element i of a line sits at i + (i >> lds_sh) (0: none)
   int32_t col_in, col_out;  // 1: consecutive lines are adjacent in memory (coalesce across lines)
   FastDiv divC, divL;       // by C and by L
   int64_t nlines;           // < 2^31
@@ -85,9 +90,11 @@ struct LinePass {
   const int64_t* in_tab;
   const int64_t* out_tab;
   const void* tw;  // stage twiddles: tw[m] = exp(-2 pi i m / L), m < L, in the pass's precision
-  // Pre-twiddle (multi-pass Stockham): element t of a line is multiplied by
+  // Pass twiddle: element t of a line is multiplied by
   // w_N^{i_twdim * t * tw_scale}, w_N^m = lo[m & (2^sh-1)] * hi[m >> sh]
-  // (FP64 tables, product in FP64, rounded to the pass's precision).
+  // (FP64 tables, product in FP64, rounded to the pass's precision),
+  // on load (pretw = 1, multi-pass Stockham) or on store (pretw = 2,
+  // four-step column/row passes).
   int32_t pretw;
   int32_t twdim;
   int32_t tw_sh;
@@ -98,14 +105,21 @@ struct LinePass {
   int32_t conj_in, conj_out;
   double scale;  // multiply on store when != 1
   int32_t load_mode, mid_mode, store_mode;
-  int32_t xcd_remap;  // workgroup order: logical tile = (bid % 8) * (grid / 8) + bid / 8
+  int32_t xcd_remap;  // host hint: single-pass launch may use the XCD-aware tile order
   // Chirp / spectrum index of a loaded element: n = sum_d i_d*ca[d] + t*cae;
   // of a stored element: n = sum_d i_d*cb[d] + k*cbe.
   int64_t chirp_n;
-  int64_t ca[4], cb[4];
-  int64_t cae, cbe;
+  int64_t ca[4], cb[4], cm[4];
+  int64_t cae, cbe, cme;  // spectrum index in the middle of a pass: n = sum_d i_d*cm[d] + k*cme
   const void* chirp;  // chirp_n entries, pass precision
   const void* bhat;   // spectrum factor (mid: L entries; store: indexed by n)
+};
+
+// Device pointers of the buffers a launch works on.
+struct BufPtrs {
+  const void* in;
+  void* out;
+  void* scratch;
 };
 
 }  // namespace fftahip

@@ -7,10 +7,12 @@
 namespace fftahip {
 
 void init_kernels(size_t lds_per_block);
-hipError_t launch_line_pass(const LinePass& p, int precision, int block, size_t lds_bytes, int64_t grid,
-                            hipStream_t stream);
-hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups,
-                                   int precision, int block, size_t lds_bytes, int64_t grid, hipStream_t stream);
+// Kernel variant a pass needs (0 plain, 1 load-side pass twiddle, 2 full).
+int pass_mode(const LinePass& p);
+// Launch ngroups independent passes (device descriptors) as one grid.
+hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
+                                   int precision, int block, const BufPtrs& bp, int remap, size_t lds_bytes,
+                                   int64_t grid, hipStream_t stream);
 hipError_t launch_gen_stage_twiddles(void* tw, int precision, int64_t L, hipStream_t stream);
 hipError_t launch_gen_twiddle_2level(double2* lo, double2* hi, int64_t N, int sh, int64_t nhi, hipStream_t stream);
 hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_t N, hipStream_t stream);

@@ -38,9 +38,14 @@ __device__ __forceinline__ double2 unit_root(int64_t m, int64_t n) {
 
 // ------------------------------------------------------------------ stages
 
+// Padded position of element i inside an LDS line (one spare slot every
+// 2^sh elements breaks the power-of-two strides of the radix stages across
+// the LDS banks).
+__device__ __forceinline__ int pad_idx(int i, int sh) { return sh ? i + (i >> sh) : i; }
+
 template <class Rt, int RAD, int EPT>
 __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl, int L, int Ns, FastDiv nsd,
-                                      int twst, const cx<Rt>* __restrict__ tw) {
+                                      int twst, const cx<Rt>* __restrict__ tw, int sh) {
   constexpr int NB = EPT / RAD;
   const int Lr = L / RAD;
   cx<Rt> v[NB][RAD];
@@ -49,7 +54,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
     const int j = jl + b * Tl;
     if (active && j < Lr) {
 #pragma unroll
-      for (int t = 0; t < RAD; ++t) v[b][t] = line[j + t * Lr];
+      for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, sh)];
       if (Ns > 1) {
         const uint32_t k = (uint32_t)j - fdiv((uint32_t)j, nsd) * (uint32_t)Ns;
         const uint32_t step = k * (uint32_t)twst;
@@ -68,7 +73,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       const uint32_t k = (uint32_t)j - q * (uint32_t)Ns;
       const int base = (int)(q * (uint32_t)Ns * RAD + k);
 #pragma unroll
-      for (int t = 0; t < RAD; ++t) line[base + t * Ns] = v[b][t];
+      for (int t = 0; t < RAD; ++t) line[pad_idx(base + t * Ns, sh)] = v[b][t];
     }
   }
   __syncthreads();
@@ -78,21 +83,22 @@ template <class Rt, int EPT>
 __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const LinePass& p,
                                       const cx<Rt>* __restrict__ tw) {
   int Ns = 1;
+  const int sh = p.lds_sh;
   for (int s = 0; s < p.nst; ++s) {
     const int r = p.rad[s];
     const FastDiv nsd = p.nsdiv[s];
     const int twst = p.twst[s];
     switch (r) {
-      case 2: stage<Rt, 2, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 3: stage<Rt, 3, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 4: stage<Rt, 4, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 5: stage<Rt, 5, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 7: stage<Rt, 7, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 8: stage<Rt, 8, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 9: stage<Rt, 9, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 11: stage<Rt, 11, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 13: stage<Rt, 13, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
-      case 16: stage<Rt, 16, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw); break;
+      case 2: stage<Rt, 2, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 3: stage<Rt, 3, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 4: stage<Rt, 4, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 5: stage<Rt, 5, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 7: stage<Rt, 7, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 8: stage<Rt, 8, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 9: stage<Rt, 9, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 11: stage<Rt, 11, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 13: stage<Rt, 13, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 16: stage<Rt, 16, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
       default: break;
     }
     Ns *= r;
@@ -107,19 +113,36 @@ struct LineShared {
   int64_t twk[kMaxLines];
   int64_t cin[kMaxLines];
   int64_t cout[kMaxLines];
+  int64_t cmid[kMaxLines];
 };
 static_assert(sizeof(LineShared) == kLineSharedBytes, "static LDS size");
 
-// Global accesses are issued in batches of kU per thread so that each thread
-// keeps kU loads in flight before it consumes any of them.
-constexpr int kU = 8;
+// Kernel variants (compile-time), so that the common passes carry no
+// register cost for operations they do not use:
+//   MODE_PLAIN  load, radix sweeps, store (optionally conjugating / scaling);
+//   MODE_PRETW  as PLAIN with the multi-pass twiddle applied on load;
+//   MODE_FULL   every fused operation, selected at run time (Bluestein).
+enum PassMode { MODE_PLAIN = 0, MODE_PRETW = 1, MODE_FULL = 2 };
 
-template <class Rt, int EPT, int BLOCK>
-__device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, cx<Rt>* lds, LineShared& sh) {
+// Global loads are issued in batches per thread (kept in flight together).
+template <int MODE>
+struct Batch {
+  static constexpr int U = MODE == MODE_PLAIN ? 8 : 4;
+};
+
+__device__ __forceinline__ double2 dmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <class Rt, int EPT, int BLOCK, int MODE>
+__device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& bp, uint32_t tile, cx<Rt>* lds,
+                                              LineShared& sh) {
+  constexpr int U = Batch<MODE>::U;
   const int tid = threadIdx.x;
   const int C = p.C;
   const int L = p.L;
   const int ldl = p.ldl;
+  const int psh = p.lds_sh;
   const int64_t line0 = (int64_t)tile * C;
   const int64_t rem = p.nlines - line0;
   const int nvalid = rem < C ? (int)rem : C;
@@ -140,33 +163,33 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
     if (p.out_tab) oo += p.out_tab[idx[3]];
     sh.in[c] = oi;
     sh.out[c] = oo;
-    sh.twk[c] = idx[p.twdim & 3];
-    sh.cin[c] = idx[0] * p.ca[0] + idx[1] * p.ca[1] + idx[2] * p.ca[2] + idx[3] * p.ca[3];
-    sh.cout[c] = idx[0] * p.cb[0] + idx[1] * p.cb[1] + idx[2] * p.cb[2] + idx[3] * p.cb[3];
+    if (MODE != MODE_PLAIN) sh.twk[c] = idx[p.twdim & 3];
+    if (MODE == MODE_FULL) {
+      sh.cin[c] = idx[0] * p.ca[0] + idx[1] * p.ca[1] + idx[2] * p.ca[2] + idx[3] * p.ca[3];
+      sh.cout[c] = idx[0] * p.cb[0] + idx[1] * p.cb[1] + idx[2] * p.cb[2] + idx[3] * p.cb[3];
+      sh.cmid[c] = idx[0] * p.cm[0] + idx[1] * p.cm[1] + idx[2] * p.cm[2] + idx[3] * p.cm[3];
+    }
   }
   __syncthreads();
 
-  const cx<Rt>* __restrict__ in = reinterpret_cast<const cx<Rt>*>(p.in);
-  cx<Rt>* __restrict__ out = reinterpret_cast<cx<Rt>*>(p.out);
+  const void* inb = p.in_buf == BUF_IN ? bp.in : (p.in_buf == BUF_OUT ? (const void*)bp.out : (const void*)bp.scratch);
+  void* outb = p.out_buf == BUF_OUT ? bp.out : bp.scratch;
+  const cx<Rt>* __restrict__ in = reinterpret_cast<const cx<Rt>*>(inb) + p.in_off;
+  cx<Rt>* __restrict__ out = reinterpret_cast<cx<Rt>*>(outb) + p.out_off;
   const cx<Rt>* __restrict__ tw = reinterpret_cast<const cx<Rt>*>(p.tw);
-  const cx<Rt>* __restrict__ chirp = reinterpret_cast<const cx<Rt>*>(p.chirp);
-  const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
   const int total = C * L;
-  const bool chirp_in = p.load_mode == LOAD_CHIRP;
   const int64_t twmask = (1ll << p.tw_sh) - 1;
+  const bool conj_in = p.conj_in != 0;
 
   // ---- load (global -> LDS), coalesced across lines or along a line
-  for (int e0 = 0; e0 < total; e0 += BLOCK * kU) {
-    cx<Rt> v[kU];
-    cx<Rt> ch[kU];
-    double2 wlo[kU], whi[kU];
-    int slot[kU];
-    bool live[kU];
+  for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
+    cx<Rt> v[U];
+    int slot[U];
+    int tt[U], cc[U];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int e = e0 + u * BLOCK + tid;
       slot[u] = -1;
-      live[u] = false;
       v[u] = {(Rt)0, (Rt)0};
       if (e < total) {
         int c, t;
@@ -177,39 +200,39 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
           c = (int)fdiv((uint32_t)e, p.divL);
           t = e - c * L;
         }
+        tt[u] = t;
+        cc[u] = c;
         if (c < nvalid) {
-          slot[u] = c * ldl + t;
-          live[u] = true;
-          if (chirp_in) {
-            const int64_t n = sh.cin[c] + (int64_t)t * p.cae;
-            live[u] = n < p.chirp_n;
-            if (live[u]) ch[u] = chirp[n];
-          }
-          if (live[u]) {
-            v[u] = in[sh.in[c] + (int64_t)t * p.ae];
-            if (p.pretw) {
-              const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
-              wlo[u] = p.tw_lo[m & twmask];
-              whi[u] = p.tw_hi[m >> p.tw_sh];
-            }
-          }
+          slot[u] = c * ldl + pad_idx(t, psh);
+          bool live = true;
+          if (MODE == MODE_FULL && p.load_mode == LOAD_CHIRP) live = sh.cin[c] + (int64_t)t * p.cae < p.chirp_n;
+          if (live) v[u] = in[sh.in[c] + (int64_t)t * p.ae];
         }
       }
     }
+    if (MODE == MODE_PLAIN) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      if (slot[u] < 0) continue;
-      cx<Rt> x = v[u];
-      if (live[u]) {
-        if (p.conj_in) x = cconj(x);
-        if (chirp_in) x = cmul(x, cconj(ch[u]));
-        if (p.pretw) {
-          const double2 w = make_double2(wlo[u].x * whi[u].x - wlo[u].y * whi[u].y,
-                                         wlo[u].x * whi[u].y + wlo[u].y * whi[u].x);
-          x = cmul(x, from_d2<Rt>(w));
-        }
+      for (int u = 0; u < U; ++u) {
+        if (slot[u] < 0) continue;
+        lds[slot[u]] = conj_in ? cconj(v[u]) : v[u];
       }
-      lds[slot[u]] = x;
+    } else {
+      // second batch: table factors (chirp, pass twiddle), consumed at once
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (slot[u] < 0) continue;
+        cx<Rt> x = conj_in ? cconj(v[u]) : v[u];
+        const int c = cc[u], t = tt[u];
+        if (MODE == MODE_FULL && p.load_mode == LOAD_CHIRP) {
+          const int64_t n = sh.cin[c] + (int64_t)t * p.cae;
+          if (n < p.chirp_n) x = cmul(x, cconj(reinterpret_cast<const cx<Rt>*>(p.chirp)[n]));
+        }
+        if (p.pretw == 1) {
+          const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
+          x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
+        }
+        lds[slot[u]] = x;
+      }
     }
   }
   __syncthreads();
@@ -221,25 +244,26 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
   cx<Rt>* myline = lds + (active ? cl : 0) * ldl;
   sweep<Rt, EPT>(myline, jl, active, p, tw);
 
-  if (p.mid_mode == MID_BHAT) {
-    for (int e0 = 0; e0 < total; e0 += BLOCK * kU) {
-      cx<Rt> f[kU];
-      int slot[kU];
+  if (MODE == MODE_FULL && p.mid_mode == MID_BHAT) {
+    const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
+    for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
+      cx<Rt> f[U];
+      int slot[U];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int e = e0 + u * BLOCK + tid;
         slot[u] = -1;
         if (e < total) {
           const int c = (int)fdiv((uint32_t)e, p.divL);
           const int t = e - c * L;
           if (c < nvalid) {
-            slot[u] = c * ldl + t;
-            f[u] = bhat[t];
+            slot[u] = c * ldl + pad_idx(t, psh);
+            f[u] = bhat[sh.cmid[c] + (int64_t)t * p.cme];
           }
         }
       }
 #pragma unroll
-      for (int u = 0; u < kU; ++u)
+      for (int u = 0; u < U; ++u)
         if (slot[u] >= 0) lds[slot[u]] = cconj(cmul(lds[slot[u]], f[u]));
     }
     __syncthreads();
@@ -249,76 +273,59 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, uint32_t tile, 
   // ---- store (LDS -> global)
   const Rt scale = (Rt)p.scale;
   const bool has_scale = p.scale != 1.0;
-  for (int e0 = 0; e0 < total; e0 += BLOCK * kU) {
-    cx<Rt> f[kU];
-    int slot[kU];
-    int64_t dst[kU];
+  const bool conj_out = p.conj_out != 0;
+  for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int e = e0 + u * BLOCK + tid;
-      slot[u] = -1;
-      if (e < total) {
-        int c, t;
-        if (p.col_out) {
-          t = (int)fdiv((uint32_t)e, p.divC);
-          c = e - t * C;
-        } else {
-          c = (int)fdiv((uint32_t)e, p.divL);
-          t = e - c * L;
+      if (e >= total) continue;
+      int c, t;
+      if (p.col_out) {
+        t = (int)fdiv((uint32_t)e, p.divC);
+        c = e - t * C;
+      } else {
+        c = (int)fdiv((uint32_t)e, p.divL);
+        t = e - c * L;
+      }
+      if (c >= nvalid) continue;
+      cx<Rt> x = lds[c * ldl + pad_idx(t, psh)];
+      if (MODE == MODE_FULL) {
+        if (p.pretw == 2) {
+          const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
+          x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
         }
-        if (c < nvalid) {
-          bool keep = true;
-          if (p.store_mode != STORE_PLAIN) {
-            const int64_t n = sh.cout[c] + (int64_t)t * p.cbe;
-            if (p.store_mode == STORE_BHAT) {
-              f[u] = bhat[n];
-            } else {
-              keep = n < p.chirp_n;
-              if (keep) f[u] = chirp[n];
-            }
-          }
-          if (keep) {
-            slot[u] = c * ldl + t;
-            dst[u] = sh.out[c] + (int64_t)t * p.be;
+        if (p.store_mode != STORE_PLAIN) {
+          const int64_t n = sh.cout[c] + (int64_t)t * p.cbe;
+          if (p.store_mode == STORE_BHAT) {
+            x = cconj(cmul(x, reinterpret_cast<const cx<Rt>*>(p.bhat)[n]));
+          } else {
+            if (n >= p.chirp_n) continue;
+            x = cconj(cmul(reinterpret_cast<const cx<Rt>*>(p.chirp)[n], x));
           }
         }
       }
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      if (slot[u] < 0) continue;
-      cx<Rt> x = lds[slot[u]];
-      if (p.store_mode == STORE_BHAT) x = cconj(cmul(x, f[u]));
-      else if (p.store_mode == STORE_CHIRP) x = cconj(cmul(f[u], x));
-      if (p.conj_out) x = cconj(x);
+      if (conj_out) x = cconj(x);
       if (has_scale) x = cscale(x, scale);
-      out[dst[u]] = x;
+      out[sh.out[c] + (int64_t)t * p.be] = x;
     }
   }
-}
-
-template <class Rt, int EPT, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) line_pass_kernel(LinePass p) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ LineShared sh;
-  uint32_t tile = blockIdx.x;
-  if (p.xcd_remap) {
-    const uint32_t per = gridDim.x >> 3;
-    tile = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  }
-  run_line_pass<Rt, EPT, BLOCK>(p, tile, reinterpret_cast<cx<Rt>*>(smem_raw), sh);
 }
 
 // Work-list launch: group g owns tiles [tile_start[g], tile_start[g+1]).
-// Every group is an independent LinePass of the same precision and block
-// size; groups may differ in length, radices and fused operations.
-template <class Rt, int EPT, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) line_pass_group_kernel(const LinePass* __restrict__ passes,
-                                                                const uint32_t* __restrict__ tile_start,
-                                                                int ngroups) {
+// Every group is an independent LinePass of the same precision, block size
+// and mode; groups may differ in length, radices and fused operations.  A
+// single-group launch with remap set visits its tiles in XCD-aware order
+// (tile = (bid % 8) * (grid / 8) + bid / 8, grid a multiple of 8) so that
+// neighbouring tiles share an L2.  At least 4 waves per SIMD (<= 128 VGPRs):
+// the passes are latency bound on global memory.
+template <class Rt, int EPT, int BLOCK, int MODE>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __restrict__ tile_start, int ngroups,
+                       BufPtrs bp, int remap) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ LineShared sh;
-  const uint32_t bid = blockIdx.x;
+  uint32_t bid = blockIdx.x;
+  if (remap) bid = (bid & 7u) * (gridDim.x >> 3) + (bid >> 3);
   int lo = 0, hi = ngroups - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -326,7 +333,7 @@ __global__ void __launch_bounds__(BLOCK) line_pass_group_kernel(const LinePass* 
     else hi = mid - 1;
   }
   const LinePass& p = passes[lo];
-  run_line_pass<Rt, EPT, BLOCK>(p, bid - tile_start[lo], reinterpret_cast<cx<Rt>*>(smem_raw), sh);
+  run_line_pass<Rt, EPT, BLOCK, MODE>(p, bp, bid - tile_start[lo], reinterpret_cast<cx<Rt>*>(smem_raw), sh);
 }
 
 // ------------------------------------------------------------------ tables
@@ -384,69 +391,65 @@ __global__ void finish_bhat_kernel(cx<Rt>* bhat, const double2* spec64, int64_t 
 // ------------------------------------------------------------------ launchers
 
 namespace {
-template <class Rt, int EPT, int BLOCK>
-hipError_t launch_line_pass_t(const LinePass& p, size_t lds_bytes, int64_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((line_pass_kernel<Rt, EPT, BLOCK>), dim3((uint32_t)grid), dim3(BLOCK), lds_bytes, stream, p);
+template <class Rt, int BLOCK, int MODE>
+void allow_lds(size_t bytes) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&line_pass_group_kernel<Rt, 16, BLOCK, MODE>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+template <class Rt, int BLOCK>
+void allow_lds_modes(size_t bytes) {
+  allow_lds<Rt, BLOCK, MODE_PLAIN>(bytes);
+  allow_lds<Rt, BLOCK, MODE_PRETW>(bytes);
+  allow_lds<Rt, BLOCK, MODE_FULL>(bytes);
+}
+template <class Rt, int BLOCK, int MODE>
+hipError_t launch_grp(const LinePass* P, const uint32_t* T, int ng, const BufPtrs& bp, int remap, size_t lds,
+                      int64_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((line_pass_group_kernel<Rt, 16, BLOCK, MODE>), dim3((uint32_t)grid), dim3(BLOCK), lds, s, P, T,
+                     ng, bp, remap);
   return hipGetLastError();
 }
-template <class Rt, int EPT, int BLOCK>
-hipError_t launch_group_t(const LinePass* passes, const uint32_t* tile_start, int ngroups, size_t lds_bytes,
-                          int64_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((line_pass_group_kernel<Rt, EPT, BLOCK>), dim3((uint32_t)grid), dim3(BLOCK), lds_bytes, stream,
-                     passes, tile_start, ngroups);
-  return hipGetLastError();
+template <class Rt, int BLOCK>
+hipError_t launch_grp_mode(const LinePass* P, const uint32_t* T, int ng, int mode, const BufPtrs& bp, int remap,
+                           size_t lds, int64_t grid, hipStream_t s) {
+  if (mode == MODE_PLAIN) return launch_grp<Rt, BLOCK, MODE_PLAIN>(P, T, ng, bp, remap, lds, grid, s);
+  if (mode == MODE_PRETW) return launch_grp<Rt, BLOCK, MODE_PRETW>(P, T, ng, bp, remap, lds, grid, s);
+  return launch_grp<Rt, BLOCK, MODE_FULL>(P, T, ng, bp, remap, lds, grid, s);
+}
+template <class Rt>
+hipError_t launch_grp_block(const LinePass* P, const uint32_t* T, int ng, int block, int mode, const BufPtrs& bp,
+                            int remap, size_t lds, int64_t grid, hipStream_t s) {
+  if (block == 256) return launch_grp_mode<Rt, 256>(P, T, ng, mode, bp, remap, lds, grid, s);
+  if (block == 512) return launch_grp_mode<Rt, 512>(P, T, ng, mode, bp, remap, lds, grid, s);
+  if (block == 1024) return launch_grp_mode<Rt, 1024>(P, T, ng, mode, bp, remap, lds, grid, s);
+  return hipErrorInvalidValue;
 }
 }  // namespace
 
-template <class Rt, int EPT, int BLOCK>
-void allow_dynamic_lds(size_t bytes) {
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&line_pass_kernel<Rt, EPT, BLOCK>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&line_pass_group_kernel<Rt, EPT, BLOCK>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  (void)hipGetLastError();
+int pass_mode(const LinePass& p) {
+  const bool full = p.load_mode != LOAD_PLAIN || p.mid_mode != MID_NONE || p.store_mode != STORE_PLAIN ||
+                    p.pretw == 2;
+  if (full) return MODE_FULL;
+  return p.pretw == 1 ? MODE_PRETW : MODE_PLAIN;
 }
 
 void init_kernels(size_t lds_per_block) {
   const size_t dyn = lds_per_block > (size_t)kLineSharedBytes ? lds_per_block - kLineSharedBytes : 0;
-  allow_dynamic_lds<float, 16, 256>(dyn);
-  allow_dynamic_lds<float, 16, 512>(dyn);
-  allow_dynamic_lds<float, 16, 1024>(dyn);
-  allow_dynamic_lds<double, 16, 256>(dyn);
-  allow_dynamic_lds<double, 16, 512>(dyn);
-  allow_dynamic_lds<double, 16, 1024>(dyn);
+  allow_lds_modes<float, 256>(dyn);
+  allow_lds_modes<float, 512>(dyn);
+  allow_lds_modes<float, 1024>(dyn);
+  allow_lds_modes<double, 256>(dyn);
+  allow_lds_modes<double, 512>(dyn);
+  allow_lds_modes<double, 1024>(dyn);
+  (void)hipGetLastError();
 }
 
-hipError_t launch_line_pass(const LinePass& p, int precision, int block, size_t lds_bytes, int64_t grid,
-                            hipStream_t stream) {
-  if (precision == 32) {
-    if (block == 256) return launch_line_pass_t<float, 16, 256>(p, lds_bytes, grid, stream);
-    if (block == 512) return launch_line_pass_t<float, 16, 512>(p, lds_bytes, grid, stream);
-    if (block == 1024) return launch_line_pass_t<float, 16, 1024>(p, lds_bytes, grid, stream);
-  } else {
-    if (block == 256) return launch_line_pass_t<double, 16, 256>(p, lds_bytes, grid, stream);
-    if (block == 512) return launch_line_pass_t<double, 16, 512>(p, lds_bytes, grid, stream);
-    if (block == 1024) return launch_line_pass_t<double, 16, 1024>(p, lds_bytes, grid, stream);
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups,
-                                   int precision, int block, size_t lds_bytes, int64_t grid, hipStream_t stream) {
-  if (precision == 32) {
-    if (block == 256) return launch_group_t<float, 16, 256>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
-    if (block == 512) return launch_group_t<float, 16, 512>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
-    if (block == 1024)
-      return launch_group_t<float, 16, 1024>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
-  } else {
-    if (block == 256)
-      return launch_group_t<double, 16, 256>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
-    if (block == 512)
-      return launch_group_t<double, 16, 512>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
-    if (block == 1024)
-      return launch_group_t<double, 16, 1024>(d_passes, d_tile_start, ngroups, lds_bytes, grid, stream);
-  }
-  return hipErrorInvalidValue;
+hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
+                                   int precision, int block, const BufPtrs& bp, int remap, size_t lds_bytes,
+                                   int64_t grid, hipStream_t stream) {
+  if (precision == 32)
+    return launch_grp_block<float>(d_passes, d_tile_start, ngroups, block, mode, bp, remap, lds_bytes, grid, stream);
+  return launch_grp_block<double>(d_passes, d_tile_start, ngroups, block, mode, bp, remap, lds_bytes, grid, stream);
 }
 
 hipError_t launch_gen_stage_twiddles(void* tw, int precision, int64_t L, hipStream_t stream) {

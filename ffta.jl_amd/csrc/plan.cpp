@@ -256,12 +256,13 @@ static bool smooth13(int64_t n) {
 
 // ------------------------------------------------------------------ length plan
 
-enum Kind { K_SINGLE = 0, K_MULTI = 1, K_BLUE_SINGLE = 2, K_BLUE_MULTI = 3 };
+enum Kind { K_SINGLE = 0, K_MULTI = 1, K_BLUE_SINGLE = 2, K_BLUE_MULTI = 3, K_BLUE_3PASS = 4 };
 static const char* kind_name(int k) {
   switch (k) {
     case K_SINGLE: return "stockham-1pass";
     case K_MULTI: return "stockham-multipass";
     case K_BLUE_SINGLE: return "bluestein-1kernel";
+    case K_BLUE_3PASS: return "bluestein-3pass";
     default: return "bluestein-multipass";
   }
 }
@@ -363,13 +364,40 @@ static int64_t min_lines(int prec) { return prec == 32 ? 16 : 8; }
 static int64_t lds_budget();
 static int64_t pass_budget() { return std::min<int64_t>(env_i64("FFTAHIP_PASS_LDS", 65536), lds_budget()); }
 static int64_t single_max(int prec) {
-  int64_t m = lds_budget() / (int64_t)csize(prec);
+  // room for the 1-in-16 LDS padding
+  int64_t m = lds_budget() / (int64_t)csize(prec) * 16 / 17;
   return std::min<int64_t>(m, 1024 * kEPT);
 }
 static int64_t pass_max(int prec) {
   const int64_t dflt = std::min<int64_t>(1024, pass_budget() / ((int64_t)csize(prec) * min_lines(prec)));
   const int64_t v = env_i64("FFTAHIP_PASS_MAX", dflt);
   return std::max<int64_t>(v, 16);
+}
+
+// Longest row of the fused Bluestein row pass (one line per workgroup).
+static int64_t row_max(int prec) {
+  return std::min<int64_t>(single_max(prec), env_i64("FFTAHIP_ROW_MAX", prec == 32 ? 8192 : 4096));
+}
+
+// Bluestein length M = M1 * M2 for the three fused passes: column passes of
+// length M1 (strided, <= pass_max) and one row pass of length M2 (<= row_max).
+static bool blue3_split(int64_t M, int prec, int64_t* M1, int64_t* M2) {
+  // prefer column passes of >= 32 points (enough work per strided tile),
+  // then the longest row that fits
+  const int64_t pmax = pass_max(prec);
+  Inner a, b;
+  for (int64_t min_m1 : {(int64_t)32, (int64_t)2}) {
+    for (int64_t m2 = std::min(M / min_m1, row_max(prec)); m2 >= 2; --m2) {
+      if (M % m2) continue;
+      const int64_t m1 = M / m2;
+      if (m1 < min_m1 || m1 > pmax) continue;
+      if (!make_inner(m2, b) || !make_inner(m1, a)) continue;
+      *M1 = m1;
+      *M2 = m2;
+      return true;
+    }
+  }
+  return false;
 }
 
 // Split M into pass factors, each <= cap and smooth; balanced greedily.
@@ -415,6 +443,16 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
       return FFTAHIP_SUCCESS;
     }
   }
+  int64_t M1, M2;
+  if (lp.blue && blue3_split(target, prec, &M1, &M2)) {
+    d.kind = K_BLUE_3PASS;
+    d.F = {M1, M2};
+    Inner a, b;
+    make_inner(M1, a);
+    make_inner(M2, b);
+    d.inner = {a, b};
+    return FFTAHIP_SUCCESS;
+  }
   std::vector<int64_t> F;
   if (!split_factors(target, pass_max(prec), F))
     return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(target));
@@ -458,16 +496,17 @@ struct LineSet {
 struct Dim {
   int64_t n, a, b, ca, cb;
   bool tw;
+  int64_t cm = 0;  // coefficient of the mid-pass spectrum index
 };
 
 enum LaunchOp { OP_PASS = 0, OP_COPY = 1 };
 struct Launch {
-  int op = OP_PASS;
+  int op = OP_PASS;  // OP_COPY: copy `bytes` from the input buffer to the output buffer
   LinePass lp;
   int block = 256;
   size_t lds = 0;
   int64_t grid = 0;
-  size_t bytes = 0;  // OP_COPY
+  size_t bytes = 0;
 };
 
 // Fill geometry: dims (<= 3 affine + optional table dim) -> LinePass fields.
@@ -476,7 +515,7 @@ static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
   std::vector<Dim> D;
   for (size_t i = 0; i < Din.size(); ++i)
     if (Din[i].n > 1 || (tab && i + 1 == Din.size())) D.push_back(Din[i]);
-  if (D.empty()) D.push_back(Dim{1, 0, 0, 0, 0, false});
+  if (D.empty()) D.push_back(Dim{1, 0, 0, 0, 0, false, 0});
   // choose slot 0 among the affine dims for coalescing
   const size_t naff = tab ? D.size() - 1 : D.size();
   int pick = -1;
@@ -485,7 +524,7 @@ static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
   for (size_t i = 0; i < naff && pick < 0; ++i)
     if (D[i].n > 1 && D[i].b == 1 && p.be != 1) pick = (int)i;
   if (pick > 0) std::swap(D[0], D[pick]);
-  std::vector<Dim> slots(4, Dim{1, 0, 0, 0, 0, false});
+  std::vector<Dim> slots(4, Dim{1, 0, 0, 0, 0, false, 0});
   for (size_t i = 0; i < naff; ++i) slots[i] = D[i];
   if (tab) slots[3] = D.back();
   int64_t total = 1;
@@ -501,10 +540,71 @@ static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
     p.b[i] = slots[i].b;
     p.ca[i] = slots[i].ca;
     p.cb[i] = slots[i].cb;
+    p.cm[i] = slots[i].cm;
     if (slots[i].tw) p.twdim = i;
   }
   p.col_in = (p.ae != 1 && slots[0].n > 1 && slots[0].a == 1) ? 1 : 0;
   p.col_out = (p.be != 1 && slots[0].n > 1 && slots[0].b == 1) ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ LDS bank model
+// 32 banks of 4 bytes; one 64-lane access takes as many cycles as the most
+// loaded bank has distinct words.  Used to pick the LDS padding of a pass.
+
+static int padded(int i, int sh) { return sh ? i + (i >> sh) : i; }
+
+static int access_cycles(const std::vector<int>& elem, int words) {
+  int cnt[32] = {0};
+  std::vector<int> seen;
+  seen.reserve(elem.size() * words);
+  for (int e : elem)
+    for (int w = 0; w < words; ++w) {
+      const int word = e * words + w;
+      if (std::find(seen.begin(), seen.end(), word) != seen.end()) continue;
+      seen.push_back(word);
+      ++cnt[word & 31];
+    }
+  int m = 0;
+  for (int b = 0; b < 32; ++b) m = std::max(m, cnt[b]);
+  return m;
+}
+
+// Cycles per element of LDS traffic for one line pass with padding sh and
+// line pitch ldl (wave 0 pattern of every stage, load and store phases).
+static double lds_cost(const Inner& in, int C, int ldl, int sh, int words, bool col_in, bool col_out) {
+  const int L = (int)in.L;
+  double cost = 0;
+  std::vector<int> idx;
+  // load / store phases
+  for (int side = 0; side < 2; ++side) {
+    const bool col = side == 0 ? col_in : col_out;
+    idx.clear();
+    for (int e = 0; e < 64 && e < C * L; ++e) {
+      int c, t;
+      if (col) { t = e / C; c = e % C; } else { c = e / L; t = e % L; }
+      idx.push_back(c * ldl + padded(t, sh));
+    }
+    if (!idx.empty()) cost += access_cycles(idx, words) / 64.0;
+  }
+  int Ns = 1;
+  for (int R : in.rad) {
+    const int Lr = L / R;
+    double sc = 0;
+    for (int t = 0; t < R; ++t) {
+      std::vector<int> rd, wr;
+      for (int tid = 0; tid < 64; ++tid) {
+        const int c = tid / in.Tl, j = tid % in.Tl;
+        if (c >= C || j >= Lr) continue;
+        const int k = j % Ns, base = (j / Ns) * Ns * R + k;
+        rd.push_back(c * ldl + padded(j + t * Lr, sh));
+        wr.push_back(c * ldl + padded(base + t * Ns, sh));
+      }
+      if (!rd.empty()) sc += access_cycles(rd, words) + access_cycles(wr, words);
+    }
+    cost += sc / (64.0 * R) * 1.0;
+    Ns *= R;
+  }
+  return cost;
 }
 
 static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
@@ -533,11 +633,52 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   if (strided) C = std::max<int64_t>(C, min_lines(prec));
   C = std::min<int64_t>(C, kMaxLines);
   C = std::min<int64_t>(C, std::max<int64_t>(1, 1024 / in.Tl));
-  int64_t ldl = in.L + (strided ? 1 : 0);
-  const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pass_budget(), ldl * esz)) : lds_budget();
-  while (C > 1 && C * ldl * esz > budget) --C;
+  const int64_t padmax = in.L + (in.L - 1) / 16 + 1;  // largest pitch any padding choice needs
+  const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pass_budget(), padmax * esz)) : lds_budget();
+  while (C > 1 && C * padmax * esz > budget) --C;
   if (C > p.nlines) C = std::max<int64_t>(1, p.nlines);
-  if (C == 1) ldl = in.L;
+  // LDS padding: env FFTAHIP_LDS_PAD (-1 = choose by the bank model)
+  const int64_t force = env_i64("FFTAHIP_LDS_PAD", -1);
+  int best_sh = 0;
+  int64_t ldl = in.L;
+  double best = 1e30;
+  const int words = (int)(esz / 4);
+  static std::mutex pad_mu;
+  static std::map<std::vector<int64_t>, std::pair<int, int64_t>> pad_cache;
+  std::vector<int64_t> key = {in.L, in.Tl, C, words, p.col_in, p.col_out, force};
+  for (int r : in.rad) key.push_back(r);
+  {
+    std::lock_guard<std::mutex> g(pad_mu);
+    auto it = pad_cache.find(key);
+    if (it != pad_cache.end()) {
+      best_sh = it->second.first;
+      ldl = it->second.second;
+      best = 0;
+    }
+  }
+  for (int sh : {0, 4, 5}) {
+    if (best == 0) break;
+    if (force >= 0 && sh != force) continue;
+    for (int extra : {0, 1}) {
+      if (C == 1 && extra) continue;
+      const int64_t pitch = padded((int)in.L - 1, sh) + 1 + extra;
+      const double cst = lds_cost(in, (int)C, (int)pitch, sh, words, p.col_in, p.col_out);
+      if (cst < best - 1e-9) {
+        best = cst;
+        best_sh = sh;
+        ldl = pitch;
+      }
+    }
+  }
+  if (force > 0 && best > 1e29) {
+    best_sh = (int)force;
+    ldl = padded((int)in.L - 1, best_sh) + 1;
+  }
+  {
+    std::lock_guard<std::mutex> g(pad_mu);
+    pad_cache[key] = {best_sh, ldl};
+  }
+  p.lds_sh = best_sh;
   const int64_t threads = C * in.Tl;
   p.C = (int32_t)C;
   p.ldl = (int32_t)ldl;
@@ -560,8 +701,22 @@ static LinePass blank_pass() {
 
 // Build the launches that transform every line of `ls` through `lp`.
 // conj: backward sign (conjugate in and out).
-static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& ls, const void* in, void* out,
-                                     void* s0, void* s1, bool conj, std::vector<Launch>& L, hipStream_t stream) {
+// A buffer a pass reads or writes: which one (BufId) and an element offset.
+struct BufRef {
+  int id;
+  int64_t off;
+};
+static void set_in(LinePass& p, BufRef r) {
+  p.in_buf = r.id;
+  p.in_off = r.off;
+}
+static void set_out(LinePass& p, BufRef r) {
+  p.out_buf = r.id;
+  p.out_off = r.off;
+}
+
+static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& ls, BufRef in, BufRef out, BufRef s0,
+                                     BufRef s1, bool conj, std::vector<Launch>& L, hipStream_t stream) {
   const int strided = (ls.es_in != 1 || ls.es_out != 1) ? 1 : 0;
   Decomp* d;
   fftahip_status st = get_decomp(lp, strided, &d);
@@ -574,8 +729,8 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
   if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE) {
     Launch la;
     LinePass p = blank_pass();
-    p.in = in;
-    p.out = out;
+    set_in(p, in);
+    set_out(p, out);
     p.ae = ls.es_in;
     p.be = ls.es_out;
     std::vector<Dim> D;
@@ -600,12 +755,95 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       p.chirp_n = lp.N;
       p.cae = 1;
       p.cbe = 1;
+      p.cme = 1;
       p.chirp = lp.chirp_ptr;
       p.bhat = lp.bhat_ptr;
     }
     set_inner(p, d->inner[0], prec, la);
     la.lp = p;
     L.push_back(la);
+    return FFTAHIP_SUCCESS;
+  }
+
+  // batch dims: the user side follows the line set, scratch holds transform b at b*M
+  auto batch_dims = [&](std::vector<Dim>& D, bool user_in, bool user_out) {
+    if (tab) {
+      D.push_back(Dim{ls.n[1], user_in ? 0 : M, user_out ? 0 : M, 0, 0, false});
+      return;
+    }
+    int64_t sstride = M;
+    for (int b = 0; b < 2; ++b) {
+      if (ls.n[b] > 1)
+        D.push_back(Dim{ls.n[b], user_in ? ls.in_str[b] : sstride, user_out ? ls.out_str[b] : sstride, 0, 0, false});
+      sstride *= ls.n[b];
+    }
+  };
+
+  if (d->kind == K_BLUE_3PASS) {
+    // M = M1*M2, positions n = M2*n1 + n2.
+    //  K1: lines n2, FFT over n1 of chirp-multiplied input, post-twiddle w_M^{n2*k1},
+    //      store S[k1*M2 + n2];
+    //  K2: rows k1 of S, FFT over n2 -> k2, times bhat[k1 + M1*k2], conjugate,
+    //      FFT over k2 -> n2, post-twiddle w_M^{k1*n2}, store in place;
+    //  K3: lines n2 of S, FFT over k1 -> n1, store conj(chirp[n] * v) at n < N.
+    const int64_t M1 = d->F[0], M2 = d->F[1];
+    const double2* tlo = nullptr;
+    const double2* thi = nullptr;
+    int tsh = 0;
+    st = two_level(T, M, &tlo, &thi, &tsh, stream);
+    if (st) return st;
+    for (int k = 0; k < 3; ++k) {
+      Launch la;
+      LinePass p = blank_pass();
+      std::vector<Dim> D;
+      p.pretw = (k < 2) ? 2 : 0;
+      p.tw_scale = 1;
+      p.tw_lo = tlo;
+      p.tw_hi = thi;
+      p.tw_sh = tsh;
+      p.chirp_n = lp.N;
+      p.chirp = lp.chirp_ptr;
+      p.bhat = lp.bhat_ptr;
+      if (k == 0) {
+        set_in(p, in);
+        set_out(p, s0);
+        p.ae = M2 * ls.es_in;
+        p.be = M2;
+        D.push_back(Dim{M2, ls.es_in, 1, 1, 0, true});
+        batch_dims(D, true, false);
+        p.load_mode = LOAD_CHIRP;
+        p.cae = M2;
+        p.conj_in = conj ? 1 : 0;
+      } else if (k == 1) {
+        set_in(p, s0);
+        set_out(p, s0);
+        p.ae = 1;
+        p.be = 1;
+        D.push_back(Dim{M1, M2, M2, 0, 0, true, 1});
+        batch_dims(D, false, false);
+        p.mid_mode = MID_BHAT;
+        p.cme = M1;
+      } else {
+        set_in(p, s0);
+        set_out(p, out);
+        p.ae = M2;
+        p.be = M2 * ls.es_out;
+        D.push_back(Dim{M2, 1, ls.es_out, 0, 1, false});
+        batch_dims(D, false, true);
+        p.store_mode = STORE_CHIRP;
+        p.cbe = M2;
+        p.conj_out = conj ? 1 : 0;
+      }
+      set_dims(p, D, tab);
+      if (k == 0) p.in_tab = tab ? ls.in_tab : nullptr;
+      if (k == 2) p.out_tab = tab ? ls.out_tab : nullptr;
+      const int64_t Lk = (k == 1) ? M2 : M1;
+      st = stage_table(T, Lk, &p.tw, stream);
+      if (st) return st;
+      set_inner(p, d->inner[k == 1 ? 1 : 0], prec, la);
+      la.lp = p;
+      L.push_back(la);
+    }
     return FFTAHIP_SUCCESS;
   }
 
@@ -627,8 +865,8 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
     const bool first = i == 0, last = i == total - 1;
     Launch la;
     LinePass p = blank_pass();
-    p.in = first ? in : (((i - 1) % 2 == 0) ? s0 : s1);
-    p.out = last ? out : ((i % 2 == 0) ? s0 : s1);
+    set_in(p, first ? in : (((i - 1) % 2 == 0) ? s0 : s1));
+    set_out(p, last ? out : ((i % 2 == 0) ? s0 : s1));
     const int64_t ein = first ? ls.es_in : 1;
     const int64_t eout = last ? ls.es_out : 1;
     p.ae = ein * (M / R);
@@ -689,20 +927,42 @@ static fftahip_status scratch_per_line(LengthPlan& lp, int strided, int64_t* ele
   if (st) return st;
   *elems = d->M;
   if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE) *nbuf = 0;
-  else if (d->kind == K_MULTI && d->F.size() == 2) *nbuf = 1;
+  else if ((d->kind == K_MULTI && d->F.size() == 2) || d->kind == K_BLUE_3PASS) *nbuf = 1;
   else *nbuf = 2;
   return FFTAHIP_SUCCESS;
 }
 
-static fftahip_status run_launches(const std::vector<Launch>& L, int prec, hipStream_t s) {
-  for (const Launch& la : L) {
+// A launch list with its descriptors resident on the device (uploaded once;
+// independent of the arrays it runs on).
+struct DevSchedule {
+  std::vector<Launch> L;
+  DevBuf passes;  // LinePass per launch
+  DevBuf zero;    // tile_start of single-group launches
+  size_t scratch_need = 0;
+};
+
+static fftahip_status upload_schedule(DevSchedule& S) {
+  std::vector<LinePass> host;
+  for (const Launch& la : S.L) host.push_back(la.lp);
+  fftahip_status st = S.passes.ensure(std::max<size_t>(1, host.size()) * sizeof(LinePass));
+  if (st) return st;
+  if ((st = S.zero.ensure(sizeof(uint32_t)))) return st;
+  if (!host.empty()) HIP_TRY(hipMemcpy(S.passes.p, host.data(), host.size() * sizeof(LinePass), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(S.zero.p, 0, sizeof(uint32_t)));
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status run_schedule(const DevSchedule& S, int prec, const BufPtrs& bp, hipStream_t s) {
+  for (size_t i = 0; i < S.L.size(); ++i) {
+    const Launch& la = S.L[i];
     if (la.op == OP_COPY) {
-      HIP_TRY(hipMemcpyAsync(la.lp.out, la.lp.in, la.bytes, hipMemcpyDeviceToDevice, s));
+      if (bp.out != bp.in) HIP_TRY(hipMemcpyAsync(bp.out, bp.in, la.bytes, hipMemcpyDeviceToDevice, s));
       continue;
     }
     if (la.grid <= 0) continue;
     if (la.grid > 0x7fffffffLL) return fail(FFTAHIP_ERR_CAPACITY, "grid too large");
-    HIP_TRY(launch_line_pass(la.lp, prec, la.block, la.lds, la.grid, s));
+    HIP_TRY(launch_line_pass_groups((const LinePass*)S.passes.p + i, (const uint32_t*)S.zero.p, 1, pass_mode(la.lp),
+                                    prec, la.block, bp, la.lp.xcd_remap, la.lds, la.grid, s));
   }
   return FFTAHIP_SUCCESS;
 }
@@ -716,7 +976,7 @@ static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s
   if (st) return st;
   st = lp.bhat.ensure((size_t)M * csize(lp.prec));
   if (st) return st;
-  DevBuf c64, bext, spec, s0, s1;
+  DevBuf c64, bext, spec, s0;
   if ((st = c64.ensure((size_t)N * sizeof(double2)))) return st;
   if ((st = bext.ensure((size_t)M * sizeof(double2)))) return st;
   if ((st = spec.ensure((size_t)M * sizeof(double2)))) return st;
@@ -732,12 +992,14 @@ static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s
   int64_t per;
   int nbuf;
   if ((st = scratch_per_line(sub, 0, &per, &nbuf))) return st;
-  if (nbuf >= 1 && (st = s0.ensure((size_t)per * 16))) return st;
-  if (nbuf >= 2 && (st = s1.ensure((size_t)per * 16))) return st;
+  if (nbuf >= 1 && (st = s0.ensure((size_t)per * nbuf * 16))) return st;
   LineSet ls;
-  std::vector<Launch> L;
-  if ((st = build_launches(sub, T64, ls, bext.p, spec.p, s0.p, s1.p, false, L, s))) return st;
-  if ((st = run_launches(L, 64, s))) return st;
+  DevSchedule S;
+  if ((st = build_launches(sub, T64, ls, BufRef{BUF_IN, 0}, BufRef{BUF_OUT, 0}, BufRef{BUF_SCRATCH, 0},
+                           BufRef{BUF_SCRATCH, per}, false, S.L, s)))
+    return st;
+  if ((st = upload_schedule(S))) return st;
+  if ((st = run_schedule(S, 64, BufPtrs{bext.p, spec.p, s0.p}, s))) return st;
   HIP_TRY(launch_finish_bhat(lp.bhat.p, (const double2*)spec.p, lp.prec, M, s));
   HIP_TRY(hipStreamSynchronize(s));
   lp.chirp_ptr = lp.chirp.p;
@@ -757,14 +1019,20 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
   if (lp.blue) {
     // smallest 5-smooth length >= 2N-1 (FFTA pads to a power of two,
     // src/algos.jl:345; any length >= 2N-1 gives the same convolution)
-    int64_t M = 2 * N - 1;
-    for (;; ++M) {
-      int64_t m = M;
+    auto smooth5 = [](int64_t v) {
       for (int q : {2, 3, 5})
-        while (m % q == 0) m /= q;
-      if (m == 1) break;
+        while (v % q == 0) v /= q;
+      return v == 1;
+    };
+    int64_t first = 0, chosen = 0;
+    for (int64_t M = 2 * N - 1; M <= 4 * (2 * N - 1) && !chosen; ++M) {
+      if (!smooth5(M)) continue;
+      if (!first) first = M;
+      Inner in;
+      int64_t a, b;
+      if ((M <= single_max(prec) && make_inner(M, in)) || blue3_split(M, prec, &a, &b)) chosen = M;
     }
-    lp.M = M;
+    lp.M = chosen ? chosen : first;
   } else {
     lp.M = N;
   }
@@ -795,6 +1063,7 @@ struct fftahip_plan_s {
   std::vector<int64_t> region;                    // sorted, 1-based
   std::vector<std::unique_ptr<LengthPlan>> lens;  // one per region dim
   Tables tables;
+  std::map<std::vector<int64_t>, std::unique_ptr<DevSchedule>> scheds;  // per array shape
   DevBuf scratch;
   DevBuf hin, hout;  // staging for execute_host
   std::string last_schedule;
@@ -824,6 +1093,7 @@ struct fftahip_batch_s {
   DevBuf tiles;
   struct Phase {
     int block;
+    int mode;
     size_t lds;
     int64_t grid;
     int ngroups;
@@ -832,8 +1102,6 @@ struct fftahip_batch_s {
   std::vector<Phase> phases;
   std::vector<LinePass> host_passes;
   std::vector<uint32_t> host_tiles;
-  const void* built_in = nullptr;
-  void* built_out = nullptr;
   hipGraphExec_t graph = nullptr;
   const void* graph_in = nullptr;
   void* graph_out = nullptr;
@@ -1026,35 +1294,21 @@ static LineSet lines_along(int ndims, const int64_t* dims, int d) {
 
 // The launches of mul!(y, p, x) for an array of shape dims: region dims in
 // ascending order, the first reading x and every later one working in y
-// (src/plan.jl:232-253).  `scratch` must hold *scratch_need bytes.
-static fftahip_status plan_schedule(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
-                                    void* scratch, std::vector<Launch>& L, size_t* scratch_need, hipStream_t s) {
+// (src/plan.jl:232-253).  The scratch buffer must hold *scratch_need bytes.
+static fftahip_status plan_schedule(fftahip_plan plan, int ndims, const int64_t* dims, std::vector<Launch>& L,
+                                    size_t* scratch_need, hipStream_t s) {
   int64_t total = 1;
   for (int i = 0; i < ndims; ++i) total *= dims[i];
   size_t need = 0;
-  for (size_t k = 0; k < plan->region.size(); ++k) {
-    const int d = (int)plan->region[k] - 1;
-    if (d >= ndims) continue;
-    LineSet ls = lines_along(ndims, dims, d);
-    int64_t per;
-    int nbuf;
-    fftahip_status st = scratch_per_line(*plan->lens[k], (ls.es_in != 1) ? 1 : 0, &per, &nbuf);
-    if (st) return st;
-    need = std::max(need, (size_t)(per * ls.count() * nbuf) * csize(plan->prec));
-  }
-  *scratch_need = need;
-  if (!scratch && need > 0) return FFTAHIP_SUCCESS;  // sizing only
-  bool wrote = false;  // d_out already holds the partial result
+  bool wrote = false;  // y already holds the partial result
   for (size_t k = 0; k < plan->region.size(); ++k) {
     const int d = (int)plan->region[k] - 1;
     if (d >= ndims) {
       // size(x, d) == 1 beyond ndims: a length-1 transform is the identity
-      if (!wrote && d_in != d_out) {
+      if (!wrote) {
         Launch cp;
         cp.op = OP_COPY;
         cp.lp = blank_pass();
-        cp.lp.in = d_in;
-        cp.lp.out = d_out;
         cp.bytes = (size_t)total * csize(plan->prec);
         L.push_back(cp);
       }
@@ -1066,13 +1320,14 @@ static fftahip_status plan_schedule(fftahip_plan plan, int ndims, const int64_t*
     int nbuf;
     fftahip_status st = scratch_per_line(*plan->lens[k], (ls.es_in != 1) ? 1 : 0, &per, &nbuf);
     if (st) return st;
-    char* s0 = (char*)scratch;
-    char* s1 = nbuf >= 2 ? s0 + (size_t)(per * ls.count()) * csize(plan->prec) : nullptr;
-    const void* src = wrote ? (const void*)d_out : d_in;
+    need = std::max(need, (size_t)(per * ls.count() * nbuf) * csize(plan->prec));
+    const BufRef src = wrote ? BufRef{BUF_OUT, 0} : BufRef{BUF_IN, 0};
     wrote = true;
-    st = build_launches(*plan->lens[k], plan->tables, ls, src, d_out, s0, s1, plan->dir == 1, L, s);
+    st = build_launches(*plan->lens[k], plan->tables, ls, src, BufRef{BUF_OUT, 0}, BufRef{BUF_SCRATCH, 0},
+                        BufRef{BUF_SCRATCH, per * ls.count()}, plan->dir == 1, L, s);
     if (st) return st;
   }
+  *scratch_need = need;
   return FFTAHIP_SUCCESS;
 }
 
@@ -1083,14 +1338,19 @@ fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims
   int64_t total = 1;
   for (int i = 0; i < ndims; ++i) total *= dims[i];
   if (total == 0) return FFTAHIP_SUCCESS;
-  std::vector<Launch> L;
-  size_t need = 0;
-  fftahip_status st = plan_schedule(plan, ndims, dims, d_in, d_out, nullptr, L, &need, s);
+  std::vector<int64_t> key(dims, dims + ndims);
+  auto it = plan->scheds.find(key);
+  if (it == plan->scheds.end()) {
+    auto S = std::make_unique<DevSchedule>();
+    fftahip_status st = plan_schedule(plan, ndims, dims, S->L, &S->scratch_need, s);
+    if (st) return st;
+    if ((st = upload_schedule(*S))) return st;
+    it = plan->scheds.emplace(key, std::move(S)).first;
+  }
+  DevSchedule& S = *it->second;
+  fftahip_status st = plan->scratch.ensure(S.scratch_need);
   if (st) return st;
-  if ((st = plan->scratch.ensure(need))) return st;
-  L.clear();
-  if ((st = plan_schedule(plan, ndims, dims, d_in, d_out, plan->scratch.p, L, &need, s))) return st;
-  return run_launches(L, plan->prec, s);
+  return run_schedule(S, plan->prec, BufPtrs{d_in, d_out, plan->scratch.p}, s);
 }
 
 fftahip_status fftahip_execute_host(fftahip_plan plan, int ndims, const int64_t* dims, const void* h_in,
@@ -1172,6 +1432,8 @@ fftahip_status fftahip_plan_destroy(fftahip_plan plan) {
 
 // ------------------------------------------------------------------ batches
 
+static fftahip_status batch_build(fftahip_batch b, hipStream_t s);
+
 fftahip_status fftahip_batch_create(fftahip_batch* batch, int precision, int64_t count, const int64_t* lengths,
                                     const int64_t* offsets, const int* directions, int64_t bluestein_cutoff) {
   if (!batch) return fail(FFTAHIP_ERR_ARGUMENT, "batch pointer is NULL");
@@ -1230,6 +1492,8 @@ fftahip_status fftahip_batch_create(fftahip_batch* batch, int precision, int64_t
     b->offtab_ptr = (const int64_t*)token(TK_OFFTAB, 0);
     b->scratch_ptr = (char*)token(TK_SCRATCH, 0);
     b->scratch.bytes = scratch_elems * csize(precision);
+    fftahip_status st = batch_build(b.get(), s);
+    if (st) return st;
     *batch = b.release();
     return FFTAHIP_SUCCESS;
   }
@@ -1239,21 +1503,22 @@ fftahip_status fftahip_batch_create(fftahip_batch* batch, int precision, int64_t
   if ((st = b->scratch.ensure(scratch_elems * csize(precision)))) return st;
   b->offtab_ptr = (const int64_t*)b->offtab.p;
   b->scratch_ptr = (char*)b->scratch.p;
+  if ((st = batch_build(b.get(), s))) return st;
   HIP_TRY(hipDeviceSynchronize());
   *batch = b.release();
   return FFTAHIP_SUCCESS;
 }
 
 // Build the per-phase work lists for the given pointers.
-static fftahip_status batch_build(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
+// Build the per-phase work lists (pointer independent; run with BufPtrs).
+static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
   b->phases.clear();
   b->host_passes.clear();
   b->host_tiles.clear();
   std::vector<std::vector<Launch>>& per_group = b->per_group;
   per_group.clear();
   size_t toff = 0;
-  char* scr = b->scratch_ptr;
-  const size_t es = csize(b->prec);
+  int64_t scr = 0;  // element offset of this group's scratch
   for (auto& g : b->groups) {
     LengthPlan& lp = *b->lens[g.N];
     LineSet ls;
@@ -1266,11 +1531,10 @@ static fftahip_status batch_build(fftahip_batch b, const void* d_in, void* d_out
     int nbuf;
     fftahip_status st = scratch_per_line(lp, 0, &per, &nbuf);
     if (st) return st;
-    void* s0 = nbuf >= 1 ? scr : nullptr;
-    void* s1 = nbuf >= 2 ? scr + (size_t)(per * ls.n[1]) * es : nullptr;
-    scr += (size_t)(per * ls.n[1] * nbuf) * es;
+    const BufRef s0{BUF_SCRATCH, scr}, s1{BUF_SCRATCH, scr + per * ls.n[1]};
+    scr += per * ls.n[1] * nbuf;
     std::vector<Launch> L;
-    st = build_launches(lp, b->tables, ls, d_in, d_out, s0, s1, g.dir == 1, L, s);
+    st = build_launches(lp, b->tables, ls, BufRef{BUF_IN, 0}, BufRef{BUF_OUT, 0}, s0, s1, g.dir == 1, L, s);
     if (st) return st;
     per_group.push_back(L);
   }
@@ -1286,17 +1550,21 @@ static fftahip_status batch_build(fftahip_batch b, const void* d_in, void* d_out
   size_t maxp = 0;
   for (auto& L : per_group) maxp = std::max(maxp, L.size());
   for (size_t ph = 0; ph < maxp; ++ph) {
+    for (int mode = 0; mode < 3; ++mode)
     for (int block : {256, 512, 1024}) {
       for (size_t cls : kLdsClass) {
         fftahip_batch_s::Phase P;
         P.block = block;
+        P.mode = mode;
         P.lds = 0;
         P.ngroups = 0;
         P.pass_off = b->host_passes.size();
         P.tile_off = b->host_tiles.size();
         uint32_t tiles = 0;
         for (auto& L : per_group) {
-          if (ph >= L.size() || L[ph].block != block || L[ph].grid <= 0 || lds_class(L[ph].lds) != cls) continue;
+          if (ph >= L.size() || L[ph].block != block || L[ph].grid <= 0 || lds_class(L[ph].lds) != cls ||
+              pass_mode(L[ph].lp) != mode)
+            continue;
           LinePass lp = L[ph].lp;
           lp.xcd_remap = 0;
           b->host_passes.push_back(lp);
@@ -1311,11 +1579,7 @@ static fftahip_status batch_build(fftahip_batch b, const void* d_in, void* d_out
       }
     }
   }
-  if (g_dry) {
-    b->built_in = d_in;
-    b->built_out = d_out;
-    return FFTAHIP_SUCCESS;
-  }
+  if (g_dry) return FFTAHIP_SUCCESS;
   fftahip_status st = b->passes.ensure(b->host_passes.size() * sizeof(LinePass));
   if (st) return st;
   if ((st = b->tiles.ensure(b->host_tiles.size() * sizeof(uint32_t)))) return st;
@@ -1325,50 +1589,38 @@ static fftahip_status batch_build(fftahip_batch b, const void* d_in, void* d_out
   if (!b->host_tiles.empty())
     HIP_TRY(hipMemcpy(b->tiles.p, b->host_tiles.data(), b->host_tiles.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
-  b->built_in = d_in;
-  b->built_out = d_out;
   return FFTAHIP_SUCCESS;
 }
 
-static fftahip_status batch_launch(fftahip_batch b, hipStream_t s) {
+static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
+  const BufPtrs bp{d_in, d_out, b->scratch_ptr};
   for (auto& P : b->phases) {
     HIP_TRY(launch_line_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
-                                    P.ngroups, b->prec, P.block, P.lds, P.grid, s));
+                                    P.ngroups, P.mode, b->prec, P.block, bp, 0, P.lds, P.grid, s));
   }
   return FFTAHIP_SUCCESS;
 }
 
 fftahip_status fftahip_batch_execute(fftahip_batch b, const void* d_in, void* d_out, void* stream) {
   if (!b) return fail(FFTAHIP_ERR_ARGUMENT, "NULL batch");
-  hipStream_t s = (hipStream_t)stream;
   if (b->count == 0) return FFTAHIP_SUCCESS;
-  if (b->built_in != d_in || b->built_out != d_out || b->phases.empty()) {
-    HIP_TRY(hipStreamSynchronize(s));
-    fftahip_status st = batch_build(b, d_in, d_out, s);
-    if (st) return st;
-  }
-  return batch_launch(b, s);
+  return batch_launch(b, d_in, d_out, (hipStream_t)stream);
 }
 
 fftahip_status fftahip_batch_execute_graph(fftahip_batch b, const void* d_in, void* d_out, void* stream) {
   if (!b) return fail(FFTAHIP_ERR_ARGUMENT, "NULL batch");
   hipStream_t s = (hipStream_t)stream;
   if (b->count == 0) return FFTAHIP_SUCCESS;
-  if (b->built_in != d_in || b->built_out != d_out || b->phases.empty()) {
-    HIP_TRY(hipStreamSynchronize(s));
-    fftahip_status st = batch_build(b, d_in, d_out, s);
-    if (st) return st;
-    if (b->graph) {
-      (void)hipGraphExecDestroy(b->graph);
-      b->graph = nullptr;
-    }
+  if (b->graph && (b->graph_in != d_in || b->graph_out != d_out)) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
   }
   if (!b->graph) {
     hipStream_t cs;
     HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    fftahip_status st = batch_launch(b, cs);
+    fftahip_status st = batch_launch(b, d_in, d_out, cs);
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (st) return st;
@@ -1379,6 +1631,8 @@ fftahip_status fftahip_batch_execute_graph(fftahip_batch b, const void* d_in, vo
       b->graph = nullptr;
       return fail(FFTAHIP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
+    b->graph_in = d_in;
+    b->graph_out = d_out;
   }
   HIP_TRY(hipGraphLaunch(b->graph, s));
   return FFTAHIP_SUCCESS;
@@ -1454,18 +1708,23 @@ static void json_ptr(std::ostringstream& os, const char* name, const void* p) {
   }
 }
 
-static void json_launch(std::ostringstream& os, const Launch& la) {
+static void json_buf(std::ostringstream& os, const char* name, int id, int64_t off, int prec) {
+  const int kind = id == BUF_IN ? TK_IN : (id == BUF_OUT ? TK_OUT : TK_SCRATCH);
+  os << "\"" << name << "\": [" << kind << ", " << off * (int64_t)csize(prec) << "]";
+}
+
+static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   const LinePass& p = la.lp;
   os << "{\"op\": " << la.op << ", \"block\": " << la.block << ", \"lds\": " << la.lds << ", \"grid\": " << la.grid
      << ", \"bytes\": " << la.bytes << ", ";
-  json_ptr(os, "in", p.in);
+  json_buf(os, "in", la.op == OP_COPY ? BUF_IN : p.in_buf, la.op == OP_COPY ? 0 : p.in_off, prec);
   os << ", ";
-  json_ptr(os, "out", p.out);
+  json_buf(os, "out", la.op == OP_COPY ? BUF_OUT : p.out_buf, la.op == OP_COPY ? 0 : p.out_off, prec);
   os << ", \"L\": " << p.L << ", \"rad\": [";
   for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.rad[i];
   os << "], \"twst\": [";
   for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.twst[i];
-  os << "], \"C\": " << p.C << ", \"Tl\": " << p.Tl << ", \"ldl\": " << p.ldl << ", \"col_in\": " << p.col_in
+  os << "], \"C\": " << p.C << ", \"Tl\": " << p.Tl << ", \"ldl\": " << p.ldl << ", \"lds_sh\": " << p.lds_sh << ", \"col_in\": " << p.col_in
      << ", \"col_out\": " << p.col_out << ", \"nlines\": " << p.nlines << ", \"n\": [" << p.n[0] << ", " << p.n[1]
      << ", " << p.n[2] << "], \"a\": [" << p.a[0] << ", " << p.a[1] << ", " << p.a[2] << ", " << p.a[3]
      << "], \"b\": [" << p.b[0] << ", " << p.b[1] << ", " << p.b[2] << ", " << p.b[3] << "], \"ae\": " << p.ae
@@ -1484,7 +1743,8 @@ static void json_launch(std::ostringstream& os, const Launch& la) {
      << ", \"load_mode\": " << p.load_mode << ", \"mid_mode\": " << p.mid_mode << ", \"store_mode\": " << p.store_mode
      << ", \"xcd_remap\": " << p.xcd_remap << ", \"chirp_n\": " << p.chirp_n << ", \"ca\": [" << p.ca[0] << ", "
      << p.ca[1] << ", " << p.ca[2] << ", " << p.ca[3] << "], \"cb\": [" << p.cb[0] << ", " << p.cb[1] << ", "
-     << p.cb[2] << ", " << p.cb[3] << "], \"cae\": " << p.cae << ", \"cbe\": " << p.cbe << ", ";
+     << p.cb[2] << ", " << p.cb[3] << "], \"cm\": [" << p.cm[0] << ", " << p.cm[1] << ", " << p.cm[2] << ", "
+     << p.cm[3] << "], \"cae\": " << p.cae << ", \"cbe\": " << p.cbe << ", \"cme\": " << p.cme << ", ";
   json_ptr(os, "chirp", p.chirp);
   os << ", ";
   json_ptr(os, "bhat", p.bhat);
@@ -1516,13 +1776,13 @@ fftahip_status fftahip_debug_schedule(int precision, int direction, int ndims, c
   std::unique_ptr<fftahip_plan_s> owner(plan);
   std::vector<Launch> L;
   size_t need = 0;
-  st = plan_schedule(plan, ndims, dims, token(TK_IN, 0), token(TK_OUT, 0), token(TK_SCRATCH, 0), L, &need, nullptr);
+  st = plan_schedule(plan, ndims, dims, L, &need, nullptr);
   if (st) return st;
   std::ostringstream os;
   os << "{\"scratch_bytes\": " << need << ", \"launches\": [";
   for (size_t i = 0; i < L.size(); ++i) {
     if (i) os << ", ";
-    json_launch(os, L[i]);
+    json_launch(os, L[i], precision);
   }
   os << "]}";
   return emit(os.str(), buf, cap, needed);
@@ -1537,7 +1797,6 @@ fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const 
   fftahip_status st = fftahip_batch_create(&b, precision, count, lengths, nullptr, directions, 73);
   if (st) return st;
   std::unique_ptr<fftahip_batch_s> owner(b);
-  if ((st = batch_build(b, token(TK_IN, 0), token(TK_OUT, 0), nullptr))) return st;
   std::ostringstream os;
   os << "{\"scratch_bytes\": " << b->scratch.bytes << ", \"offtab\": [";
   for (size_t i = 0; i < b->host_offtab.size(); ++i) os << (i ? ", " : "") << b->host_offtab[i];
@@ -1546,7 +1805,7 @@ fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const 
     os << (gi ? ", " : "") << "[";
     for (size_t i = 0; i < b->per_group[gi].size(); ++i) {
       if (i) os << ", ";
-      json_launch(os, b->per_group[gi][i]);
+      json_launch(os, b->per_group[gi][i], precision);
     }
     os << "]";
   }

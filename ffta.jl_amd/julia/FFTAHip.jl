@@ -1,0 +1,141 @@
+"""
+    FFTAHip
+
+FFTA.jl's complex `fft`/`bfft` plan interface running on an MI355X through the
+C ABI of `libfftahip.so` (include/fftahip.h).  Mirrors FFTA's
+`src/plan.jl`: `plan_fft`/`plan_bfft` (`:71-100`), `size(p[, i])` (`:36-47`),
+`mul!(y, p, x)` (`:157-285`) and `p * x` (`:374-384`), with the same error
+types and messages (`DimensionMismatch`, `DomainError`).  Arrays are Julia
+`Array{ComplexF32/ComplexF64}` (column-major, interleaved complex), which is
+exactly the library's layout; `mul!` stages them through device memory.
+
+Set `ENV["FFTAHIP_LIB"]` to the path of `libfftahip.so` before loading.
+"""
+module FFTAHip
+
+using AbstractFFTs: AbstractFFTs
+using LinearAlgebra: LinearAlgebra
+
+const libfftahip = get(ENV, "FFTAHIP_LIB", joinpath(@__DIR__, "..", "lib", "libfftahip.so"))
+
+const FFT_FORWARD = Cint(-1)
+const FFT_BACKWARD = Cint(1)
+const DEFAULT_BLUESTEIN_CUTOFF = 73
+
+# status codes of fftahip_status
+const SUCCESS = Cint(0)
+const ERR_DIMENSION_MISMATCH = Cint(1)
+const ERR_ARGUMENT = Cint(2)
+const ERR_DOMAIN = Cint(3)
+
+last_error() = unsafe_string(ccall((:fftahip_last_error, libfftahip), Cstring, ()))
+
+function check(st::Cint, arg=nothing)
+    st == SUCCESS && return nothing
+    msg = last_error()
+    st == ERR_DIMENSION_MISMATCH && throw(DimensionMismatch(msg))
+    st == ERR_DOMAIN && throw(DomainError(arg, msg))
+    st == ERR_ARGUMENT && throw(ArgumentError(msg))
+    error("fftahip: " * msg)
+end
+
+precision_of(::Type{ComplexF32}) = Cint(32)
+precision_of(::Type{ComplexF64}) = Cint(64)
+
+mutable struct FFTAHipPlan{T<:Union{ComplexF32,ComplexF64},N} <: AbstractFFTs.Plan{T}
+    handle::Ptr{Cvoid}
+    region::Union{Int,Vector{Int}}
+    dir::Cint
+end
+
+function FFTAHipPlan{T,N}(x::AbstractArray{T}, region, dir::Cint, cutoff::Int) where {T,N}
+    dims = Int64[size(x)...]
+    reg = Int64[sort(collect(region))...]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fftahip_plan_create, libfftahip), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint, Ptr{Int64}, Cint, Ptr{Int64}, Int64),
+                h, precision_of(T), dir, length(dims), dims, length(reg), reg, cutoff))
+    r = N == 1 ? Int(reg[1]) : Vector{Int}(reg)
+    p = FFTAHipPlan{T,N}(h[], r, dir)
+    finalizer(p) do q
+        q.handle == C_NULL || ccall((:fftahip_plan_destroy, libfftahip), Cint, (Ptr{Cvoid},), q.handle)
+        q.handle = C_NULL
+    end
+    return p
+end
+
+# src/plan.jl:71-100
+_plan(x::AbstractArray{T}, region, dir; BLUESTEIN_CUTOFF=DEFAULT_BLUESTEIN_CUTOFF, _kwargs...) where {T} =
+    FFTAHipPlan{T,length(region)}(x, region, dir, BLUESTEIN_CUTOFF)
+AbstractFFTs.plan_fft(x::AbstractArray{T}, region; kwargs...) where {T<:Union{ComplexF32,ComplexF64}} =
+    _plan(x, region, FFT_FORWARD; kwargs...)
+AbstractFFTs.plan_bfft(x::AbstractArray{T}, region; kwargs...) where {T<:Union{ComplexF32,ComplexF64}} =
+    _plan(x, region, FFT_BACKWARD; kwargs...)
+
+# src/plan.jl:36-47
+function Base.size(p::FFTAHipPlan{T,N}, i::Int) where {T,N}
+    out = Ref{Int64}(0)
+    check(ccall((:fftahip_plan_size, libfftahip), Cint, (Ptr{Cvoid}, Cint, Ptr{Int64}), p.handle, i, out), i)
+    return Int(out[])
+end
+Base.size(p::FFTAHipPlan{T,N}) where {T,N} = ntuple(i -> size(p, i), Val(N))
+
+# src/plan.jl:157-285: argument checks in the library, then the transform
+function LinearAlgebra.mul!(y::StridedArray{T}, p::FFTAHipPlan{T}, x::StridedArray{T}) where {T}
+    Base.require_one_based_indexing(x, y)
+    reg = Int64[(p.region isa Int ? [p.region] : p.region)...]
+    dx = Int64[size(x)...]
+    dy = Int64[size(y)...]
+    check(ccall((:fftahip_plan_check, libfftahip), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Int64}, Cint, Ptr{Int64}, Cint, Ptr{Int64}),
+                p.handle, length(reg), reg, length(dx), dx, length(dy), dy))
+    GC.@preserve x y begin
+        check(ccall((:fftahip_execute_host, libfftahip), Cint,
+                    (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}),
+                    p.handle, length(dx), dx, pointer(x), pointer(y)))
+    end
+    return y
+end
+
+# src/plan.jl:374-384
+function Base.:*(p::FFTAHipPlan{T}, x::AbstractArray{T}) where {T}
+    y = similar(x)
+    LinearAlgebra.mul!(y, p, x)
+    return y
+end
+
+"""
+    fft_batch!(y, x, lengths; directions) -> y
+
+Many independent 1-D transforms of uneven lengths packed back to back in `x`
+(one work-list launch per pass on the device).
+"""
+function fft_batch!(y::Vector{T}, x::Vector{T}, lengths::Vector{Int}, directions::Vector{Cint};
+                    BLUESTEIN_CUTOFF=DEFAULT_BLUESTEIN_CUTOFF) where {T<:Union{ComplexF32,ComplexF64}}
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    L = Int64.(lengths)
+    check(ccall((:fftahip_batch_create, libfftahip), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cint}, Int64),
+                b, precision_of(T), length(L), L, C_NULL, directions, BLUESTEIN_CUTOFF))
+    try
+        nbytes = sizeof(x)
+        din = Ref{Ptr{Cvoid}}(C_NULL); dout = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:fftahip_malloc, libfftahip), Cint, (Ptr{Ptr{Cvoid}}, Csize_t), din, nbytes))
+        check(ccall((:fftahip_malloc, libfftahip), Cint, (Ptr{Ptr{Cvoid}}, Csize_t), dout, nbytes))
+        GC.@preserve x y begin
+            check(ccall((:fftahip_memcpy_htod, libfftahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
+                        din[], pointer(x), nbytes))
+            check(ccall((:fftahip_batch_execute, libfftahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                        b[], din[], dout[], C_NULL))
+            check(ccall((:fftahip_memcpy_dtoh, libfftahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
+                        pointer(y), dout[], nbytes))
+        end
+        ccall((:fftahip_free, libfftahip), Cint, (Ptr{Cvoid},), din[])
+        ccall((:fftahip_free, libfftahip), Cint, (Ptr{Cvoid},), dout[])
+    finally
+        ccall((:fftahip_batch_destroy, libfftahip), Cint, (Ptr{Cvoid},), b[])
+    end
+    return y
+end
+
+end # module

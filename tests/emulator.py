@@ -16,7 +16,7 @@ import numpy as np
 
 TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_OFFTAB = 1, 2, 3, 4, 5, 6, 7, 8, 10
 EPT = 16
-STATIC_LDS = 5 * 8 * 256
+STATIC_LDS = 6 * 8 * 256
 
 
 class ScheduleError(AssertionError):
@@ -119,7 +119,9 @@ class Emulator:
         L, C, Tl = p["L"], p["C"], p["Tl"]
         if C < 1 or C > 256 or C * Tl > p["block"]:
             raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
-        if p["ldl"] < L or p["lds"] != C * p["ldl"] * self.esize:
+        sh = p["lds_sh"]
+        need = L + ((L - 1) >> sh if sh else 0)
+        if p["ldl"] < need or p["lds"] != C * p["ldl"] * self.esize:
             raise ScheduleError("LDS size mismatch")
         if p["lds"] + STATIC_LDS > lds_per_block:
             raise ScheduleError(f"LDS {p['lds']} + static exceeds {lds_per_block}")
@@ -153,6 +155,7 @@ class Emulator:
             oout = oout + self.offtab[base + idx[3]]
         cin = sum(idx[d] * ca[d] for d in range(4))
         cout = sum(idx[d] * cb[d] for d in range(4))
+        cmid = sum(idx[d] * p["cm"][d] for d in range(4))
         t = np.arange(L, dtype=np.int64)
         src, so = self._buf(p["in"])
         dst, do = self._buf(p["out"])
@@ -180,7 +183,7 @@ class Emulator:
             n = cin[:, None] + t[None, :] * p["cae"]
             nn = np.where(valid_in, n, 0)
             v = np.where(valid_in, v * np.conj(chirp[nn]), 0)
-        if p["pretw"]:
+        if p["pretw"] == 1:
             M = p["tw_lo"][1]
             m = idx[p["twdim"]][:, None] * t[None, :] * p["tw_scale"]
             if m.max() >= M:
@@ -188,10 +191,17 @@ class Emulator:
             v = v * np.exp(-2j * np.pi * m / M)
         v = np.fft.fft(v, axis=1)
         if p["mid_mode"] == 1:
-            if bhat is None or bhat.size != L:
-                raise ScheduleError("mid spectrum table does not match L")
-            v = np.conj(v * bhat[None, :])
+            n = cmid[:, None] + t[None, :] * p["cme"]
+            if bhat is None or n.max() >= bhat.size:
+                raise ScheduleError("mid spectrum index out of range")
+            v = np.conj(v * bhat[n])
             v = np.fft.fft(v, axis=1)
+        if p["pretw"] == 2:
+            M = p["tw_lo"][1]
+            m = idx[p["twdim"]][:, None] * t[None, :] * p["tw_scale"]
+            if m.max() >= M:
+                raise ScheduleError("pass twiddle exponent out of range")
+            v = v * np.exp(-2j * np.pi * m / M)
         addr_out = do + oout[:, None] + t[None, :] * p["be"]
         valid_out = np.ones_like(addr_out, dtype=bool)
         if p["store_mode"] == 1:
