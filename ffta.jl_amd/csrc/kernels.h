@@ -17,6 +17,7 @@ hipError_t launch_gen_stage_twiddles(void* tw, int precision, int64_t L, hipStre
 hipError_t launch_gen_twiddle_2level(double2* lo, double2* hi, int64_t N, int sh, int64_t nhi, hipStream_t stream);
 hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_t N, hipStream_t stream);
 hipError_t launch_gen_chirp_ext(double2* bext, const double2* chirp64, int64_t N, int64_t M, hipStream_t stream);
+hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, hipStream_t stream);
 hipError_t launch_finish_bhat(void* bhat, const double2* spec64, int precision, int64_t M, hipStream_t stream);
 
 }  // namespace fftahip

@@ -311,21 +311,22 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   }
 }
 
-// Work-list launch: group g owns tiles [tile_start[g], tile_start[g+1]).
-// Every group is an independent LinePass of the same precision, block size
-// and mode; groups may differ in length, radices and fused operations.  A
-// single-group launch with remap set visits its tiles in XCD-aware order
-// (tile = (bid % 8) * (grid / 8) + bid / 8, grid a multiple of 8) so that
-// neighbouring tiles share an L2.  At least 4 waves per SIMD (<= 128 VGPRs):
-// the passes are latency bound on global memory.
+// Work-list launch: group g owns tiles [tile_start[g], tile_start[g+1]) of
+// ntiles.  Every group is an independent LinePass of the same precision,
+// block size and mode; groups may differ in length, radices and fused
+// operations.  Tiles are visited in XCD-aware order (workgroups are dealt
+// round-robin to the 8 XCDs, so tile = (bid % 8) * (grid / 8) + bid / 8 with
+// grid a multiple of 8 gives each XCD a contiguous range of tiles and its
+// L2 the neighbouring lines).  At least 4 waves per SIMD (<= 128 VGPRs): the
+// passes are latency bound on global memory.
 template <class Rt, int EPT, int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __restrict__ tile_start, int ngroups,
-                       BufPtrs bp, int remap) {
+                       BufPtrs bp, uint32_t ntiles) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ LineShared sh;
-  uint32_t bid = blockIdx.x;
-  if (remap) bid = (bid & 7u) * (gridDim.x >> 3) + (bid >> 3);
+  const uint32_t bid = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (bid >= ntiles) return;
   int lo = 0, hi = ngroups - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -388,6 +389,16 @@ __global__ void finish_bhat_kernel(cx<Rt>* bhat, const double2* spec64, int64_t 
   bhat[i] = {(Rt)(spec64[i].x * inv), (Rt)(spec64[i].y * inv)};
 }
 
+// dst[k1*M2 + k2] = src[k1 + M1*k2]: the spectrum in the order the fused
+// Bluestein row pass reads it (row k1 contiguous).
+template <class Rt>
+__global__ void permute_rows_kernel(cx<Rt>* dst, const cx<Rt>* src, int64_t M1, int64_t M2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M1 * M2) return;
+  const int64_t k1 = i / M2, k2 = i - k1 * M2;
+  dst[i] = src[k1 + M1 * k2];
+}
+
 // ------------------------------------------------------------------ launchers
 
 namespace {
@@ -405,8 +416,10 @@ void allow_lds_modes(size_t bytes) {
 template <class Rt, int BLOCK, int MODE>
 hipError_t launch_grp(const LinePass* P, const uint32_t* T, int ng, const BufPtrs& bp, int remap, size_t lds,
                       int64_t grid, hipStream_t s) {
-  hipLaunchKernelGGL((line_pass_group_kernel<Rt, 16, BLOCK, MODE>), dim3((uint32_t)grid), dim3(BLOCK), lds, s, P, T,
-                     ng, bp, remap);
+  (void)remap;
+  const uint32_t padded = (uint32_t)((grid + 7) & ~7ll);
+  hipLaunchKernelGGL((line_pass_group_kernel<Rt, 16, BLOCK, MODE>), dim3(padded), dim3(BLOCK), lds, s, P, T, ng, bp,
+                     (uint32_t)grid);
   return hipGetLastError();
 }
 template <class Rt, int BLOCK>
@@ -484,6 +497,17 @@ hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_
 hipError_t launch_gen_chirp_ext(double2* bext, const double2* chirp64, int64_t N, int64_t M, hipStream_t stream) {
   hipLaunchKernelGGL(gen_chirp_ext_kernel, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, stream, bext, chirp64,
                      N, M);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, hipStream_t stream) {
+  const int64_t blocks = (M1 * M2 + 255) / 256;
+  if (precision == 32)
+    hipLaunchKernelGGL(permute_rows_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<cx<float>*>(dst), reinterpret_cast<const cx<float>*>(src), M1, M2);
+  else
+    hipLaunchKernelGGL(permute_rows_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<cx<double>*>(dst), reinterpret_cast<const cx<double>*>(src), M1, M2);
   return hipGetLastError();
 }
 

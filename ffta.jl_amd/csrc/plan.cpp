@@ -85,7 +85,7 @@ static DevInfo& dev_info() {
 static thread_local bool g_dry = false;
 static thread_local int64_t g_dry_lds = 0;
 enum TokenKind { TK_STAGE = 1, TK_TWLO = 2, TK_TWHI = 3, TK_CHIRP = 4, TK_BHAT = 5, TK_IN = 6, TK_OUT = 7,
-                 TK_SCRATCH = 8, TK_OFFTAB = 10 };
+                 TK_SCRATCH = 8, TK_BHATROWS = 9, TK_OFFTAB = 10 };
 static void* token(int kind, int64_t v) {
   return (void*)(uintptr_t)((1ull << 60) | ((uint64_t)kind << 52) | (uint64_t)v);
 }
@@ -256,13 +256,14 @@ static bool smooth13(int64_t n) {
 
 // ------------------------------------------------------------------ length plan
 
-enum Kind { K_SINGLE = 0, K_MULTI = 1, K_BLUE_SINGLE = 2, K_BLUE_MULTI = 3, K_BLUE_3PASS = 4 };
+enum Kind { K_SINGLE = 0, K_MULTI = 1, K_BLUE_SINGLE = 2, K_BLUE_MULTI = 3, K_BLUE_3PASS = 4, K_MIXED = 5 };
 static const char* kind_name(int k) {
   switch (k) {
     case K_SINGLE: return "stockham-1pass";
     case K_MULTI: return "stockham-multipass";
     case K_BLUE_SINGLE: return "bluestein-1kernel";
     case K_BLUE_3PASS: return "bluestein-3pass";
+    case K_MIXED: return "bluestein-line+stockham";
     default: return "bluestein-multipass";
   }
 }
@@ -282,10 +283,14 @@ struct LengthPlan {
   int64_t cutoff = 73;
   std::vector<FftaNode> ffta;
   bool blue = false;  // N has a prime factor > 13
-  int64_t M = 1;      // Bluestein length (pow2 >= 2N-1) or N
-  DevBuf chirp, bhat;
+  bool mixed = false; // N = S*Q: Q through an in-LDS Bluestein line, S through Stockham passes
+  int64_t S = 1;      // 13-smooth part of N
+  int64_t bq = 1;     // length that goes through Bluestein (N, or Q when mixed)
+  int64_t M = 1;      // padded Bluestein length (>= 2*bq-1), or N
+  DevBuf chirp, bhat, bhat_rows;
   const void* chirp_ptr = nullptr;  // chirp.p, or a token in dry-run mode
   const void* bhat_ptr = nullptr;
+  const void* bhat_rows_ptr = nullptr;  // spectrum permuted for the fused row pass (M = M1*M2)
   bool blue_ready = false;
   std::map<int, Decomp> decomp;  // key: 0 contiguous lines, 1 strided lines
 };
@@ -431,6 +436,26 @@ static bool split_factors(int64_t M, int64_t cap, std::vector<int64_t>& F) {
 static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
   d = Decomp();
   const int prec = lp.prec;
+  if (lp.mixed) {
+    // pass 0: Q-point DFTs as Bluestein lines of M in LDS; then Stockham over S
+    d.kind = K_MIXED;
+    d.M = lp.N;
+    std::vector<int64_t> FS;
+    if (lp.S <= pass_max(prec)) FS = {lp.S};
+    else if (!split_factors(lp.S, pass_max(prec), FS))
+      return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(lp.S));
+    d.F = {lp.bq};
+    Inner qb;
+    if (!make_inner(lp.M, qb)) return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no line schedule for Bluestein length");
+    d.inner = {qb};
+    for (int64_t f : FS) {
+      Inner in;
+      make_inner(f, in);
+      d.F.push_back(f);
+      d.inner.push_back(in);
+    }
+    return FFTAHIP_SUCCESS;
+  }
   const int64_t target = lp.blue ? lp.M : lp.N;
   d.M = target;
   const int64_t single_cap = strided ? pass_max(prec) : single_max(prec);
@@ -752,7 +777,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       p.load_mode = LOAD_CHIRP;
       p.mid_mode = MID_BHAT;
       p.store_mode = STORE_CHIRP;
-      p.chirp_n = lp.N;
+      p.chirp_n = lp.bq;
       p.cae = 1;
       p.cbe = 1;
       p.cme = 1;
@@ -801,7 +826,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       p.tw_lo = tlo;
       p.tw_hi = thi;
       p.tw_sh = tsh;
-      p.chirp_n = lp.N;
+      p.chirp_n = lp.bq;
       p.chirp = lp.chirp_ptr;
       p.bhat = lp.bhat_ptr;
       if (k == 0) {
@@ -819,10 +844,11 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
         set_out(p, s0);
         p.ae = 1;
         p.be = 1;
-        D.push_back(Dim{M1, M2, M2, 0, 0, true, 1});
+        D.push_back(Dim{M1, M2, M2, 0, 0, true, M2});
         batch_dims(D, false, false);
         p.mid_mode = MID_BHAT;
-        p.cme = M1;
+        p.cme = 1;
+        p.bhat = lp.bhat_rows_ptr;
       } else {
         set_in(p, s0);
         set_out(p, out);
@@ -891,7 +917,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       }
     }
     set_dims(p, D, tab);
-    st = stage_table(T, R, &p.tw, stream);
+    st = stage_table(T, d->inner[s].L, &p.tw, stream);
     if (st) return st;
     if (Ns > 1) {
       p.pretw = 1;
@@ -903,7 +929,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
     p.conj_in = (first && conj) ? 1 : 0;
     p.conj_out = (last && conj) ? 1 : 0;
     if (blue) {
-      p.chirp_n = lp.N;
+      p.chirp_n = lp.bq;
       p.chirp = lp.chirp_ptr;
       p.bhat = lp.bhat_ptr;
       p.cae = M / R;
@@ -911,6 +937,20 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       if (first) p.load_mode = LOAD_CHIRP;
       if (i == np - 1) p.store_mode = STORE_BHAT;
       if (last) p.store_mode = STORE_CHIRP;
+    }
+    if (d->kind == K_MIXED && s == 0) {
+      // the R = Q point DFT of every line as a Bluestein line of lp.M points
+      // in LDS: positions t < Q of the line are loaded and stored
+      p.chirp_n = lp.bq;
+      p.chirp = lp.chirp_ptr;
+      p.bhat = lp.bhat_ptr;
+      p.load_mode = LOAD_CHIRP;
+      p.mid_mode = MID_BHAT;
+      p.store_mode = STORE_CHIRP;
+      p.cae = 1;
+      p.cme = 1;
+      p.cbe = 1;
+      for (int u = 0; u < 4; ++u) p.ca[u] = p.cb[u] = p.cm[u] = 0;
     }
     set_inner(p, d->inner[s], prec, la);
     la.lp = p;
@@ -927,7 +967,7 @@ static fftahip_status scratch_per_line(LengthPlan& lp, int strided, int64_t* ele
   if (st) return st;
   *elems = d->M;
   if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE) *nbuf = 0;
-  else if ((d->kind == K_MULTI && d->F.size() == 2) || d->kind == K_BLUE_3PASS) *nbuf = 1;
+  else if (((d->kind == K_MULTI || d->kind == K_MIXED) && d->F.size() == 2) || d->kind == K_BLUE_3PASS) *nbuf = 1;
   else *nbuf = 2;
   return FFTAHIP_SUCCESS;
 }
@@ -971,7 +1011,7 @@ static fftahip_status run_schedule(const DevSchedule& S, int prec, const BufPtrs
 
 // bhat = FFT_M(b_ext) / M computed with an FP64 plan of the same machinery.
 static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s) {
-  const int64_t N = lp.N, M = lp.M;
+  const int64_t N = lp.bq, M = lp.M;
   fftahip_status st = lp.chirp.ensure((size_t)N * csize(lp.prec));
   if (st) return st;
   st = lp.bhat.ensure((size_t)M * csize(lp.prec));
@@ -1001,6 +1041,12 @@ static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s
   if ((st = upload_schedule(S))) return st;
   if ((st = run_schedule(S, 64, BufPtrs{bext.p, spec.p, s0.p}, s))) return st;
   HIP_TRY(launch_finish_bhat(lp.bhat.p, (const double2*)spec.p, lp.prec, M, s));
+  int64_t M1, M2;
+  if (!lp.mixed && blue3_split(M, lp.prec, &M1, &M2)) {
+    if ((st = lp.bhat_rows.ensure((size_t)M * csize(lp.prec)))) return st;
+    HIP_TRY(launch_permute_rows(lp.bhat_rows.p, lp.bhat.p, lp.prec, M1, M2, s));
+    lp.bhat_rows_ptr = lp.bhat_rows.p;
+  }
   HIP_TRY(hipStreamSynchronize(s));
   lp.chirp_ptr = lp.chirp.p;
   lp.bhat_ptr = lp.bhat.p;
@@ -1016,14 +1062,40 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
   lp.ffta.clear();
   ffta_nodes(lp.ffta, N, cutoff, 1, 1);
   lp.blue = !smooth13(N);
+  lp.mixed = false;
+  lp.S = 1;
+  lp.bq = N;
+  auto smooth5 = [](int64_t v) {
+    for (int q : {2, 3, 5})
+      while (v % q == 0) v /= q;
+    return v == 1;
+  };
   if (lp.blue) {
-    // smallest 5-smooth length >= 2N-1 (FFTA pads to a power of two,
-    // src/algos.jl:345; any length >= 2N-1 gives the same convolution)
-    auto smooth5 = [](int64_t v) {
-      for (int q : {2, 3, 5})
-        while (v % q == 0) v /= q;
-      return v == 1;
-    };
+    // N = S * Q with S the 13-smooth part: when Q's Bluestein line fits in
+    // LDS, run Q as a Bluestein pass and S as Stockham passes (FFTA splits
+    // such lengths the same way, src/callgraph.jl:102-124, with Bluestein at
+    // the prime leaf); otherwise Bluestein on the whole length.
+    int64_t S = 1, rest = N;
+    for (int q : {2, 3, 5, 7, 11, 13})
+      while (rest % q == 0) { rest /= q; S *= q; }
+    if (S > 1) {
+      for (int64_t M = 2 * rest - 1; M <= 4 * (2 * rest - 1); ++M) {
+        if (!smooth5(M)) continue;
+        Inner in;
+        if (M <= single_max(prec) && make_inner(M, in)) {
+          lp.mixed = true;
+          lp.S = S;
+          lp.bq = rest;
+          lp.M = M;
+        }
+        break;
+      }
+    }
+  }
+  if (lp.blue && !lp.mixed) {
+    // smallest 5-smooth length >= 2N-1 that runs in one kernel or three fused
+    // passes (FFTA pads to a power of two, src/algos.jl:345; any length >=
+    // 2N-1 gives the same convolution)
     int64_t first = 0, chosen = 0;
     for (int64_t M = 2 * N - 1; M <= 4 * (2 * N - 1) && !chosen; ++M) {
       if (!smooth5(M)) continue;
@@ -1033,7 +1105,7 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
       if ((M <= single_max(prec) && make_inner(M, in)) || blue3_split(M, prec, &a, &b)) chosen = M;
     }
     lp.M = chosen ? chosen : first;
-  } else {
+  } else if (!lp.blue) {
     lp.M = N;
   }
   return FFTAHIP_SUCCESS;
@@ -1041,8 +1113,9 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
 
 static fftahip_status prepare_length(LengthPlan& lp, Tables& T, hipStream_t s) {
   if (lp.blue && g_dry) {
-    lp.chirp_ptr = token(TK_CHIRP, lp.N);
+    lp.chirp_ptr = token(TK_CHIRP, lp.bq);
     lp.bhat_ptr = token(TK_BHAT, lp.M);
+    lp.bhat_rows_ptr = token(TK_BHATROWS, lp.M);
     lp.blue_ready = true;
     return FFTAHIP_SUCCESS;
   }

@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
-TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_OFFTAB = 1, 2, 3, 4, 5, 6, 7, 8, 10
+TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
 EPT = 16
 STATIC_LDS = 6 * 8 * 256
 
@@ -167,6 +167,12 @@ class Emulator:
         if p["bhat"] is not None:
             M = p["bhat"][1]
             bhat = bhat_of(p["chirp_n"], M)
+            if p["bhat"][0] == TK_BHATROWS:
+                # rows of the fused Bluestein row pass: [k1*M2 + k2] = bhat[k1 + M1*k2], M2 = L
+                M2 = L
+                M1 = M // M2
+                k1, k2 = np.divmod(np.arange(M), M2)
+                bhat = bhat[k1 + M1 * k2]
         if p["load_mode"] == 1:
             n = cin[:, None] + t[None, :] * p["cae"]
             valid_in = n < p["chirp_n"]
