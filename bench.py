@@ -67,6 +67,85 @@ def batched_uneven(count: int = 20000, seed: int = 0, lo: int = 16, hi: int = 65
     return out, dirs
 
 
+def serving_catalogue():
+    """Request shapes of the serving configuration (fixed catalogues, so that a
+    warm plan cache covers them): 1-D powers of two 64..4096; 12 composite
+    lengths log-spaced in [1e3, 1e6]; 8 primes log-spaced up to 2^20; 2-D
+    squares 256..2048."""
+    pow2 = [(1 << k,) for k in range(6, 13)]
+    comp = []
+    for v in np.round(np.logspace(3, 6, 12)).astype(np.int64):
+        v = int(v) + (int(v) % 2)  # even, hence composite
+        comp.append((v,))
+    primes = _primes_upto(1 << 20)
+    pr = []
+    for v in np.round(np.logspace(np.log10(1 << 10), np.log10(1 << 20), 8)).astype(np.int64):
+        j = min(np.searchsorted(primes, v), primes.size - 1)
+        pr.append((int(primes[j]),))
+    squares = [(n, n) for n in (256, 512, 1024, 2048)]
+    return pow2, comp, pr, squares
+
+
+def serving_trace(rate: float, seconds: float, seed: int = 0):
+    """Poisson arrivals at `rate` req/s: 70% pow2, 20% composite, 5% prime,
+    5% 2-D; 80% ComplexF32 / 20% ComplexF64; fft/bfft 50/50."""
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * seconds))
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    pow2, comp, pr, squares = serving_catalogue()
+    cat = rng.uniform(0, 1, n)
+    shapes = []
+    for c in cat:
+        if c < 0.70:
+            shapes.append(pow2[rng.integers(len(pow2))])
+        elif c < 0.90:
+            shapes.append(comp[rng.integers(len(comp))])
+        elif c < 0.95:
+            shapes.append(pr[rng.integers(len(pr))])
+        else:
+            shapes.append(squares[rng.integers(len(squares))])
+    prec = np.where(rng.uniform(0, 1, n) < 0.8, 32, 64)
+    dirs = np.where(rng.uniform(0, 1, n) < 0.5, -1, 1)
+    return t - t[0], shapes, prec, dirs
+
+
+def run_serving(torch, srv, rate: float = 50000.0, seconds: float = 0.5, seed: int = 0, ring: int = 8):
+    """Replay the serving trace against a Server with device-resident payloads."""
+    t, shapes, prec, dirs = serving_trace(rate, seconds, seed)
+    bufs = {}
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    for sh, p in set(zip(shapes, prec.tolist())):
+        n = int(np.prod(sh))
+        dt = torch.complex64 if p == 32 else torch.complex128
+        x = torch.randn(n, dtype=dt, device="cuda", generator=gen)
+        bufs[(sh, p)] = (x, [torch.empty_like(x) for _ in range(ring)])
+    # warm the plan cache: one request per (shape, precision, direction)
+    warm = []
+    for key in set(zip(shapes, prec.tolist(), dirs.tolist())):
+        sh, p, d = key
+        x, ys = bufs[(sh, p)]
+        warm.append(srv.submit(p, d, sh, x.data_ptr(), ys[0].data_ptr()))
+    for w in warm:
+        srv.wait(w)
+    srv.drain()
+    srv.reset_stats()
+    ins, outs, count = [], [], {}
+    for sh, p in zip(shapes, prec.tolist()):
+        x, ys = bufs[(sh, p)]
+        k = count.get((sh, p), 0)
+        count[(sh, p)] = k + 1
+        ins.append(x.data_ptr())
+        outs.append(ys[k % ring].data_ptr())
+    dims3 = [list(sh) + [0] * (3 - len(sh)) for sh in shapes]
+    tickets = srv.replay(t, prec.tolist(), dirs.tolist(), dims3, ins, outs)
+    srv.drain()
+    st = srv.stats()
+    st["requests"] = len(tickets)
+    st["offered_rate"] = rate
+    return st
+
+
 def flops_of(lengths) -> float:
     L = np.asarray(lengths, dtype=np.float64)
     return float(np.sum(np.where(L > 1, 5.0 * L * np.log2(np.maximum(L, 1)), 0.0)))
@@ -129,6 +208,7 @@ def main(argv=None) -> int:
     ap.add_argument("--count", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
+    ap.add_argument("--serve-seconds", type=float, default=0.5, help="length of the serving replay (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -206,6 +286,20 @@ def main(argv=None) -> int:
                   "bound": "2e-6*log2(N) vs FP64 oracle"},
         "plan_seconds": round(plan_s, 2),
     }
+    if args.serve_seconds > 0:
+        from fftahip import Server
+
+        srv = Server(pool_bytes=64 << 30, nstreams=4)
+        st = run_serving(torch, srv, rate=50000.0, seconds=args.serve_seconds)
+        srv.close()
+        line["serving"] = {
+            "workload": "Poisson 50k req/s: 70% 1-D pow2 64..4096, 20% composite 1e3..1e6, 5% prime <= 2^20, "
+                        "5% 2-D squares 256..2048; 80% ComplexF32; device-resident payloads; 64 GB pool",
+            "requests": st["requests"], "completed": st["completed"], "rejected": st["rejected"],
+            "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4), "max_ms": round(st["max_ms"], 4),
+            "throughput_req_s": round(st["requests_per_s"], 1), "payload_GBps": round(st["payload_gbps"], 2),
+            "pool_peak_bytes": st["pool_peak_bytes"],
+        }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lengths, dirs)
     print(json.dumps(line), flush=True)

@@ -1404,6 +1404,22 @@ static fftahip_status plan_schedule(fftahip_plan plan, int ndims, const int64_t*
   return FFTAHIP_SUCCESS;
 }
 
+// Shape schedule of a plan (built and uploaded on first use of the shape).
+static fftahip_status plan_shape_schedule(fftahip_plan plan, int ndims, const int64_t* dims, DevSchedule** out,
+                                          hipStream_t s) {
+  std::vector<int64_t> key(dims, dims + ndims);
+  auto it = plan->scheds.find(key);
+  if (it == plan->scheds.end()) {
+    auto S = std::make_unique<DevSchedule>();
+    fftahip_status st = plan_schedule(plan, ndims, dims, S->L, &S->scratch_need, s);
+    if (st) return st;
+    if ((st = upload_schedule(*S))) return st;
+    it = plan->scheds.emplace(key, std::move(S)).first;
+  }
+  *out = it->second.get();
+  return FFTAHIP_SUCCESS;
+}
+
 fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
                                void* stream) {
   if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
@@ -1895,3 +1911,24 @@ fftahip_status fftahip_debug_fastdiv(uint32_t d, const uint32_t* n, uint32_t* q,
 }
 
 }  // extern "C"
+
+// Library-internal entry points for the serving loop (declared in
+// plan_internal.h; C++ linkage, outside the extern "C" blocks).
+namespace fftahip {
+fftahip_status plan_scratch_bytes(fftahip_plan plan, int ndims, const int64_t* dims, size_t* bytes) {
+  DevSchedule* S;
+  fftahip_status st = plan_shape_schedule(plan, ndims, dims, &S, nullptr);
+  if (st) return st;
+  *bytes = S->scratch_need;
+  return FFTAHIP_SUCCESS;
+}
+fftahip_status plan_execute_scratch(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
+                                    void* scratch, hipStream_t s) {
+  DevSchedule* S;
+  fftahip_status st = plan_shape_schedule(plan, ndims, dims, &S, s);
+  if (st) return st;
+  return run_schedule(*S, plan->prec, BufPtrs{d_in, d_out, scratch}, s);
+}
+fftahip_status set_error(fftahip_status s, const std::string& msg) { return fail(s, msg); }
+}  // namespace fftahip
+

@@ -16,6 +16,7 @@ Reference correspondence:
 Errors map to DimensionMismatch, ArgumentError and DomainError with the
 reference's messages.
 """
+from .server import Server  # noqa: F401
 from .plan import (  # noqa: F401
     FFT_FORWARD,
     FFT_BACKWARD,

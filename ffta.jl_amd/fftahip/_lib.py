@@ -48,6 +48,19 @@ SIGNATURES = {
     "fftahip_batch_execute_graph": (_st, [_vp, _vp, _vp, _vp]),
     "fftahip_batch_describe": (_st, [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "fftahip_batch_destroy": (_st, [_vp]),
+    "fftahip_server_create": (_st, [ctypes.POINTER(_vp), ctypes.c_int64, ctypes.c_int]),
+    "fftahip_server_submit": (_st, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, _vp, _vp, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_uint64)]),
+    "fftahip_server_replay": (_st, [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int), _i64p, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "fftahip_server_test": (_st, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "fftahip_server_wait": (_st, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    "fftahip_server_drain": (_st, [_vp]),
+    "fftahip_server_stats_get": (_st, [_vp, _vp]),
+    "fftahip_server_reset_stats": (_st, [_vp]),
+    "fftahip_server_destroy": (_st, [_vp]),
     "fftahip_malloc": (_st, [ctypes.POINTER(_vp), ctypes.c_size_t]),
     "fftahip_free": (_st, [_vp]),
     "fftahip_memcpy_htod": (_st, [_vp, _vp, ctypes.c_size_t]),
@@ -63,6 +76,15 @@ SIGNATURES = {
     "fftahip_debug_fastdiv": (_st, [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64]),
 }
+
+class ServerStats(ctypes.Structure):
+    """fftahip_server_stats (include/fftahip.h)."""
+    _fields_ = [("completed", ctypes.c_int64), ("rejected", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("pending", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("requests_per_s", ctypes.c_double), ("payload_gbps", ctypes.c_double),
+                ("pool_peak_bytes", ctypes.c_int64), ("plans", ctypes.c_int64)]
+
 
 _lib = None
 

@@ -120,6 +120,49 @@ fftahip_status fftahip_batch_execute_graph(fftahip_batch batch, const void* d_in
 fftahip_status fftahip_batch_describe(fftahip_batch batch, char* buf, size_t cap, size_t* needed);
 fftahip_status fftahip_batch_destroy(fftahip_batch batch);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop with continuous batching (no reference counterpart: FFTA
+ * is a library call; this serves many fft/bfft calls concurrently).  One
+ * dispatcher thread admits submitted requests in arrival order against a
+ * device memory pool of pool_bytes (allocated once at creation; plan scratch
+ * and host-payload staging come from it), as soon as memory and a stream
+ * slot are free, and runs them on `nstreams` non-blocking HIP streams so
+ * that their work overlaps.  Plans are cached per (precision, direction,
+ * shape).  A request larger than the whole pool fails with
+ * FFTAHIP_ERR_CAPACITY.  Payloads are device pointers, or host pointers when
+ * host_buffers != 0 (staged through the pool).  Latency runs from arrival
+ * (submission, or the scheduled time for replay) to completion. */
+typedef struct fftahip_server_s* fftahip_server;
+
+typedef struct {
+  int64_t completed, rejected, failed, pending;
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double seconds;          /* first arrival to last completion */
+  double requests_per_s;   /* completed / seconds */
+  double payload_gbps;     /* (input + output bytes) / seconds */
+  int64_t pool_peak_bytes;
+  int64_t plans;
+} fftahip_server_stats;
+
+fftahip_status fftahip_server_create(fftahip_server* server, int64_t pool_bytes, int nstreams);
+/* One request: the array of shape dims (column-major), region = all dims. */
+fftahip_status fftahip_server_submit(fftahip_server server, int precision, int direction, int ndims,
+                                     const int64_t* dims, const void* in, void* out, int host_buffers,
+                                     uint64_t* ticket);
+/* Submit `count` requests, request i at t_offset[i] seconds after the call
+ * (a load generator for trace replay); dims holds 3 entries per request. */
+fftahip_status fftahip_server_replay(fftahip_server server, int64_t count, const double* t_offset,
+                                     const int* precision, const int* direction, const int* ndims,
+                                     const int64_t* dims, void* const* in, void* const* out, int host_buffers,
+                                     uint64_t* tickets);
+/* *done = 1 once the request finished (its status is then returned). */
+fftahip_status fftahip_server_test(fftahip_server server, uint64_t ticket, int* done);
+fftahip_status fftahip_server_wait(fftahip_server server, uint64_t ticket, double* latency_ms);
+fftahip_status fftahip_server_drain(fftahip_server server);
+fftahip_status fftahip_server_stats_get(fftahip_server server, fftahip_server_stats* stats);
+fftahip_status fftahip_server_reset_stats(fftahip_server server);
+fftahip_status fftahip_server_destroy(fftahip_server server);
+
 /* ---------------------------------------------------------------- device helpers
  * Minimal memory helpers so that a host language without a GPU array type
  * (the Julia wrapper, plain C) can run the device entry points. */
