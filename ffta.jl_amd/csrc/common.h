@@ -74,7 +74,6 @@ struct LinePass {
   int32_t L;
   int32_t nst;
   int32_t rad[kMaxStages];
-  int32_t twst[kMaxStages];       // L / (Ns * radix) per stage
   FastDiv nsdiv[kMaxStages];      // by Ns (product of earlier radices) per stage
   int32_t C;    // lines per workgroup
   int32_t Tl;   // threads per line (C*Tl <= block, C <= kMaxLines)
@@ -89,7 +88,7 @@ struct LinePass {
   int64_t ae, be;
   const int64_t* in_tab;
   const int64_t* out_tab;
-  const void* tw;  // stage twiddles: tw[m] = exp(-2 pi i m / L), m < L, in the pass's precision
+  const void* tw;  // per-stage twiddles of this radix schedule (kernels.hip, stage()), pass precision
   // Pass twiddle: element t of a line is multiplied by
   // w_N^{i_twdim * t * tw_scale}, w_N^m = lo[m & (2^sh-1)] * hi[m >> sh]
   // (FP64 tables, product in FP64, rounded to the pass's precision),
@@ -113,6 +112,12 @@ struct LinePass {
   int64_t cae, cbe, cme;  // spectrum index in the middle of a pass: n = sum_d i_d*cm[d] + k*cme
   const void* chirp;  // chirp_n entries, pass precision
   const void* bhat;   // spectrum factor (mid: L entries; store: indexed by n)
+};
+
+// Radix schedule of an in-LDS line (for table generation).
+struct RadixList {
+  int32_t nst;
+  int32_t rad[kMaxStages];
 };
 
 // Device pointers of the buffers a launch works on.

@@ -43,23 +43,41 @@ __device__ __forceinline__ double2 unit_root(int64_t m, int64_t n) {
 // the LDS banks).
 __device__ __forceinline__ int pad_idx(int i, int sh) { return sh ? i + (i >> sh) : i; }
 
+// One radix-RAD Stockham stage over a line in LDS.  Stage twiddles come from
+// a per-stage table: stage s (Ns = product of earlier radices) keeps
+// w_{Ns*RAD}^{t*k} for k < Ns, t = 1..RAD-1 at tw[(Ns-1) + k*(RAD-1) + t-1],
+// so one butterfly reads its RAD-1 factors contiguously.  LDS positions are
+// padded (i + (i >> sh)); when a stride is a multiple of 2^sh the padded
+// positions of a butterfly are an arithmetic sequence and are stepped
+// instead of recomputed.
 template <class Rt, int RAD, int EPT>
 __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl, int L, int Ns, FastDiv nsd,
-                                      int twst, const cx<Rt>* __restrict__ tw, int sh) {
+                                      const cx<Rt>* __restrict__ tw, int sh) {
   constexpr int NB = EPT / RAD;
   const int Lr = L / RAD;
+  const int pmask = sh ? (1 << sh) - 1 : 0;
+  const bool rstep = (Lr & pmask) == 0;
+  const bool wstep = (Ns & pmask) == 0;
+  const int rstride = Lr + (sh ? (Lr >> sh) : 0);
+  const int wstride = Ns + (sh ? (Ns >> sh) : 0);
   cx<Rt> v[NB][RAD];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = jl + b * Tl;
     if (active && j < Lr) {
+      if (rstep) {
+        const int p0 = pad_idx(j, sh);
 #pragma unroll
-      for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, sh)];
+        for (int t = 0; t < RAD; ++t) v[b][t] = line[p0 + t * rstride];
+      } else {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, sh)];
+      }
       if (Ns > 1) {
         const uint32_t k = (uint32_t)j - fdiv((uint32_t)j, nsd) * (uint32_t)Ns;
-        const uint32_t step = k * (uint32_t)twst;
+        const cx<Rt>* __restrict__ w = tw + (Ns - 1) + k * (RAD - 1) - 1;
 #pragma unroll
-        for (int t = 1; t < RAD; ++t) v[b][t] = cmul(v[b][t], tw[t * step]);
+        for (int t = 1; t < RAD; ++t) v[b][t] = cmul(v[b][t], w[t]);
       }
       dft_any<Rt, RAD>(v[b]);
     }
@@ -72,8 +90,14 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       const uint32_t q = fdiv((uint32_t)j, nsd);
       const uint32_t k = (uint32_t)j - q * (uint32_t)Ns;
       const int base = (int)(q * (uint32_t)Ns * RAD + k);
+      if (wstep) {
+        const int p0 = pad_idx(base, sh);
 #pragma unroll
-      for (int t = 0; t < RAD; ++t) line[pad_idx(base + t * Ns, sh)] = v[b][t];
+        for (int t = 0; t < RAD; ++t) line[p0 + t * wstride] = v[b][t];
+      } else {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) line[pad_idx(base + t * Ns, sh)] = v[b][t];
+      }
     }
   }
   __syncthreads();
@@ -87,18 +111,17 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
   for (int s = 0; s < p.nst; ++s) {
     const int r = p.rad[s];
     const FastDiv nsd = p.nsdiv[s];
-    const int twst = p.twst[s];
     switch (r) {
-      case 2: stage<Rt, 2, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 3: stage<Rt, 3, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 4: stage<Rt, 4, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 5: stage<Rt, 5, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 7: stage<Rt, 7, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 8: stage<Rt, 8, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 9: stage<Rt, 9, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 11: stage<Rt, 11, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 13: stage<Rt, 13, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
-      case 16: stage<Rt, 16, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, twst, tw, sh); break;
+      case 2: stage<Rt, 2, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 3: stage<Rt, 3, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 4: stage<Rt, 4, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 5: stage<Rt, 5, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 7: stage<Rt, 7, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 8: stage<Rt, 8, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 9: stage<Rt, 9, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 11: stage<Rt, 11, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 13: stage<Rt, 13, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+      case 16: stage<Rt, 16, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
       default: break;
     }
     Ns *= r;
@@ -339,11 +362,24 @@ line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __re
 
 // ------------------------------------------------------------------ tables
 
-// tw[m] = exp(-2 pi i m / L), m < L
+// Per-stage twiddles of a radix schedule: stage s (Ns = product of earlier
+// radices, radix R) holds w_{Ns*R}^{t*k} for k < Ns, t = 1..R-1 at
+// (Ns-1) + k*(R-1) + t-1; L-1 entries in all.
 template <class Rt>
-__global__ void gen_stage_twiddles_kernel(cx<Rt>* tw, int64_t L) {
+__global__ void gen_stage_twiddles_kernel(cx<Rt>* tw, RadixList rl) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < L) tw[m] = from_d2<Rt>(unit_root(m, L));
+  int64_t Ns = 1;
+  for (int s = 0; s < rl.nst; ++s) {
+    const int64_t R = rl.rad[s];
+    const int64_t lo = Ns - 1, hi = Ns * R - 1;
+    if (m >= lo && m < hi) {
+      const int64_t r = m - lo;
+      const int64_t k = r / (R - 1), t = r - k * (R - 1) + 1;
+      tw[m] = from_d2<Rt>(unit_root(t * k, Ns * R));
+      return;
+    }
+    Ns *= R;
+  }
 }
 
 // lo[m] = w_N^m for m < 2^sh; hi[h] = w_N^{h * 2^sh} for h < nhi
@@ -465,14 +501,14 @@ hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_t
   return launch_grp_block<double>(d_passes, d_tile_start, ngroups, block, mode, bp, remap, lds_bytes, grid, stream);
 }
 
-hipError_t launch_gen_stage_twiddles(void* tw, int precision, int64_t L, hipStream_t stream) {
+hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t L, hipStream_t stream) {
   const int64_t blocks = (L + 255) / 256;
   if (precision == 32)
     hipLaunchKernelGGL(gen_stage_twiddles_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                       reinterpret_cast<cx<float>*>(tw), L);
+                       reinterpret_cast<cx<float>*>(tw), rl);
   else
     hipLaunchKernelGGL(gen_stage_twiddles_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                       reinterpret_cast<cx<double>*>(tw), L);
+                       reinterpret_cast<cx<double>*>(tw), rl);
   return hipGetLastError();
 }
 

@@ -297,24 +297,32 @@ struct LengthPlan {
 
 struct Tables {
   int prec = 32;
-  std::map<int64_t, std::unique_ptr<DevBuf>> stage;               // L -> tw[L]
+  std::map<std::vector<int64_t>, std::unique_ptr<DevBuf>> stage;  // (L, radices) -> per-stage twiddles
   std::map<int64_t, std::pair<std::unique_ptr<DevBuf>, int>> two;  // M -> (lo|hi, sh)
 };
 
 static size_t csize(int prec) { return prec == 32 ? 8 : 16; }
 
-static fftahip_status stage_table(Tables& T, int64_t L, const void** out, hipStream_t s) {
+// Per-stage twiddle table of an in-LDS radix schedule (cached per schedule).
+static fftahip_status stage_table(Tables& T, const Inner& in, const void** out, hipStream_t s) {
+  std::vector<int64_t> key = {in.L};
+  for (int r : in.rad) key.push_back(r);
   if (g_dry) {
-    *out = token(TK_STAGE, L);
+    *out = token(TK_STAGE, in.L);
     return FFTAHIP_SUCCESS;
   }
-  auto it = T.stage.find(L);
+  auto it = T.stage.find(key);
   if (it == T.stage.end()) {
     auto buf = std::make_unique<DevBuf>();
-    fftahip_status st = buf->ensure((size_t)L * csize(T.prec));
+    const int64_t n = std::max<int64_t>(1, in.L - 1);
+    fftahip_status st = buf->ensure((size_t)n * csize(T.prec));
     if (st) return st;
-    HIP_TRY(launch_gen_stage_twiddles(buf->p, T.prec, L, s));
-    it = T.stage.emplace(L, std::move(buf)).first;
+    RadixList rl;
+    std::memset(&rl, 0, sizeof(rl));
+    rl.nst = (int32_t)in.rad.size();
+    for (size_t i = 0; i < in.rad.size(); ++i) rl.rad[i] = in.rad[i];
+    HIP_TRY(launch_gen_stage_twiddles(buf->p, T.prec, rl, n, s));
+    it = T.stage.emplace(key, std::move(buf)).first;
   }
   *out = it->second->p;
   return FFTAHIP_SUCCESS;
@@ -640,12 +648,10 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
     if (s < p.nst) {
       const int r = in.rad[s];
       p.rad[s] = r;
-      p.twst[s] = (int32_t)(in.L / (Ns * r));
       p.nsdiv[s] = make_fastdiv((uint32_t)Ns);
       Ns *= r;
     } else {
       p.rad[s] = 1;
-      p.twst[s] = 0;
       p.nsdiv[s] = make_fastdiv(1);
     }
   }
@@ -769,7 +775,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       if (D.empty()) D.push_back(Dim{1, 0, 0, 0, 0, false});
     }
     set_dims(p, D, tab);
-    st = stage_table(T, d->F[0], &p.tw, stream);
+    st = stage_table(T, d->inner[0], &p.tw, stream);
     if (st) return st;
     p.conj_in = conj ? 1 : 0;
     p.conj_out = conj ? 1 : 0;
@@ -863,8 +869,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       set_dims(p, D, tab);
       if (k == 0) p.in_tab = tab ? ls.in_tab : nullptr;
       if (k == 2) p.out_tab = tab ? ls.out_tab : nullptr;
-      const int64_t Lk = (k == 1) ? M2 : M1;
-      st = stage_table(T, Lk, &p.tw, stream);
+      st = stage_table(T, d->inner[k == 1 ? 1 : 0], &p.tw, stream);
       if (st) return st;
       set_inner(p, d->inner[k == 1 ? 1 : 0], prec, la);
       la.lp = p;
@@ -917,7 +922,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       }
     }
     set_dims(p, D, tab);
-    st = stage_table(T, d->inner[s].L, &p.tw, stream);
+    st = stage_table(T, d->inner[s], &p.tw, stream);
     if (st) return st;
     if (Ns > 1) {
       p.pretw = 1;
@@ -1055,6 +1060,31 @@ static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s
   return FFTAHIP_SUCCESS;
 }
 
+// Bluestein length for an in-LDS line: among 5-smooth M in [2n-1, 1.3(2n-1)]
+// that fit one workgroup, the one with the least LDS work, M * (stages + 1.5)
+// (a slightly longer M with fewer, larger radices is often cheaper).  0 if
+// none fits.
+static int64_t best_line_length(int64_t n, int prec) {
+  const int64_t lo = 2 * n - 1;
+  const int64_t hi = std::min<int64_t>(single_max(prec), lo + (lo * 3) / 10);
+  int64_t best = 0;
+  double best_cost = 1e300;
+  for (int64_t M = lo; M <= hi; ++M) {
+    int64_t m = M;
+    for (int q : {2, 3, 5})
+      while (m % q == 0) m /= q;
+    if (m != 1) continue;
+    Inner in;
+    if (!make_inner(M, in)) continue;
+    const double cost = (double)M * ((double)in.rad.size() + 1.5);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = M;
+    }
+  }
+  return best;
+}
+
 static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t cutoff) {
   lp.prec = prec;
   lp.N = N;
@@ -1079,16 +1109,12 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
     for (int q : {2, 3, 5, 7, 11, 13})
       while (rest % q == 0) { rest /= q; S *= q; }
     if (S > 1) {
-      for (int64_t M = 2 * rest - 1; M <= 4 * (2 * rest - 1); ++M) {
-        if (!smooth5(M)) continue;
-        Inner in;
-        if (M <= single_max(prec) && make_inner(M, in)) {
-          lp.mixed = true;
-          lp.S = S;
-          lp.bq = rest;
-          lp.M = M;
-        }
-        break;
+      const int64_t M = best_line_length(rest, prec);
+      if (M) {
+        lp.mixed = true;
+        lp.S = S;
+        lp.bq = rest;
+        lp.M = M;
       }
     }
   }
@@ -1096,13 +1122,13 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
     // smallest 5-smooth length >= 2N-1 that runs in one kernel or three fused
     // passes (FFTA pads to a power of two, src/algos.jl:345; any length >=
     // 2N-1 gives the same convolution)
-    int64_t first = 0, chosen = 0;
+    int64_t chosen = best_line_length(N, prec);
+    int64_t first = 0;
     for (int64_t M = 2 * N - 1; M <= 4 * (2 * N - 1) && !chosen; ++M) {
       if (!smooth5(M)) continue;
       if (!first) first = M;
-      Inner in;
       int64_t a, b;
-      if ((M <= single_max(prec) && make_inner(M, in)) || blue3_split(M, prec, &a, &b)) chosen = M;
+      if (blue3_split(M, prec, &a, &b)) chosen = M;
     }
     lp.M = chosen ? chosen : first;
   } else if (!lp.blue) {
@@ -1811,8 +1837,6 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   json_buf(os, "out", la.op == OP_COPY ? BUF_OUT : p.out_buf, la.op == OP_COPY ? 0 : p.out_off, prec);
   os << ", \"L\": " << p.L << ", \"rad\": [";
   for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.rad[i];
-  os << "], \"twst\": [";
-  for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.twst[i];
   os << "], \"C\": " << p.C << ", \"Tl\": " << p.Tl << ", \"ldl\": " << p.ldl << ", \"lds_sh\": " << p.lds_sh << ", \"col_in\": " << p.col_in
      << ", \"col_out\": " << p.col_out << ", \"nlines\": " << p.nlines << ", \"n\": [" << p.n[0] << ", " << p.n[1]
      << ", " << p.n[2] << "], \"a\": [" << p.a[0] << ", " << p.a[1] << ", " << p.a[2] << ", " << p.a[3]

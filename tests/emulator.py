@@ -37,49 +37,82 @@ def bhat_of(n: int, m: int) -> np.ndarray:
     return np.fft.fft(ext) / m
 
 
-def emulate_sweep(L: int, rad, Tl: int, rng=None) -> None:
-    """Replay the kernel's stage loop for one line; raise if any output slot is
-    missed or written twice, or if the result is not the DFT."""
-    rng = rng or np.random.default_rng(0)
-    x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
-    tw = np.exp(-2j * np.pi * np.arange(L) / L)
-    line = x.copy()
+def stage_table(L: int, rad) -> np.ndarray:
+    """Per-stage twiddles as the device table holds them: stage s (Ns = product
+    of earlier radices, radix R) stores w_{Ns*R}^{t*k}, k < Ns, t = 1..R-1, at
+    (Ns-1) + k*(R-1) + t-1."""
+    tab = np.zeros(max(1, L - 1), dtype=np.complex128)
     Ns = 1
     for R in rad:
-        twst = L // (Ns * R)
+        for k in range(Ns):
+            t = np.arange(1, R)
+            tab[(Ns - 1) + k * (R - 1) + t - 1] = np.exp(-2j * np.pi * ((t * k) % (Ns * R)) / (Ns * R))
+        Ns *= R
+    return tab
+
+
+def _pad(i, sh):
+    return i + (i >> sh) if sh else i
+
+
+def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None) -> None:
+    """Replay the kernel's stage loop for one line with its exact LDS
+    positions (padding, stepped addressing) and table indexing; raise if an
+    output slot is missed or written twice, or if the result is not the DFT."""
+    rng = rng or np.random.default_rng(0)
+    x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
+    tw = stage_table(L, rad)
+    P = _pad(L - 1, sh) + 1
+    lds = np.full(P, np.nan + 0j)
+    lds[_pad(np.arange(L), sh)] = x
+    pmask = (1 << sh) - 1 if sh else 0
+    Ns = 1
+    for R in rad:
         Lr = L // R
         NB = EPT // R
         if NB < 1:
             raise ScheduleError(f"radix {R} exceeds {EPT} registers")
         if Tl * NB < Lr:
             raise ScheduleError(f"L={L} radix {R}: {Tl} threads x {NB} butterflies < {Lr}")
-        new = np.zeros(L, dtype=np.complex128)
-        hits = np.zeros(L, dtype=np.int64)
+        rstep, wstep = (Lr & pmask) == 0, (Ns & pmask) == 0
+        rstride = Lr + ((Lr >> sh) if sh else 0)
+        wstride = Ns + ((Ns >> sh) if sh else 0)
+        new = np.full(P, np.nan + 0j)
+        hits = np.zeros(P, dtype=np.int64)
+        t = np.arange(R)
         for b in range(NB):
             j = np.arange(Tl) + b * Tl
             j = j[j < Lr]
             if j.size == 0:
                 continue
-            t = np.arange(R)
-            v = line[j[:, None] + t[None, :] * Lr]
+            if rstep:
+                src = _pad(j, sh)[:, None] + t[None, :] * rstride
+            else:
+                src = _pad(j[:, None] + t[None, :] * Lr, sh)
+            v = lds[src]
             k = j % Ns
-            idx = t[None, :] * k[:, None] * twst
-            if idx.max(initial=0) >= L:
-                raise ScheduleError("stage twiddle index out of range")
-            v = v * tw[idx]
+            if Ns > 1:
+                widx = (Ns - 1) + k[:, None] * (R - 1) + t[None, 1:] - 1
+                if widx.max() >= tw.size:
+                    raise ScheduleError("stage twiddle index out of range")
+                v[:, 1:] = v[:, 1:] * tw[widx]
             V = np.fft.fft(v, axis=1)
             base = (j // Ns) * Ns * R + k
-            dst = base[:, None] + t[None, :] * Ns
-            if dst.max() >= L:
+            if wstep:
+                dst = _pad(base, sh)[:, None] + t[None, :] * wstride
+            else:
+                dst = _pad(base[:, None] + t[None, :] * Ns, sh)
+            if dst.max() >= P:
                 raise ScheduleError("stage write out of range")
             new[dst] = V
             np.add.at(hits, dst.ravel(), 1)
-        if not np.all(hits == 1):
+        if not np.all(hits[_pad(np.arange(L), sh)] == 1) or hits.sum() != L:
             raise ScheduleError(f"L={L} stage radix {R}: slots written {np.unique(hits)} times")
-        line = new
+        lds = new
         Ns *= R
     if Ns != L:
         raise ScheduleError(f"radices {rad} multiply to {Ns}, not {L}")
+    line = lds[_pad(np.arange(L), sh)]
     ref = np.fft.fft(x)
     err = np.linalg.norm(line - ref) / max(np.linalg.norm(ref), 1e-300)
     if err > 1e-12:
@@ -129,9 +162,9 @@ class Emulator:
             raise ScheduleError("grid does not cover all lines")
         if p["xcd_remap"] and p["grid"] % 8:
             raise ScheduleError("XCD remap needs grid % 8 == 0")
-        key = (L, tuple(p["rad"]), Tl)
+        key = (L, tuple(p["rad"]), Tl, p["lds_sh"])
         if key not in self.sweeps_checked:
-            emulate_sweep(L, p["rad"], Tl)
+            emulate_sweep(L, p["rad"], Tl, p["lds_sh"])
             self.sweeps_checked.add(key)
         if p["tw"] is None or p["tw"][0] != TK_STAGE or p["tw"][1] != L:
             raise ScheduleError("stage twiddle table does not match L")
