@@ -80,6 +80,7 @@ struct LinePass {
   int32_t ldl;  // LDS row pitch in elements (>= padded L)
   int32_t lds_sh;  // LDS padding: element i of a line sits at i + (i >> lds_sh) (0: none)
   int32_t col_in, col_out;  // 1: consecutive lines are adjacent in memory (coalesce across lines)
+  int32_t fuse_in, fuse_out; // 1: first stage loads from / last stage stores to global memory directly
   FastDiv divC, divL;       // by C and by L
   int64_t nlines;           // < 2^31
   int64_t n[3];

@@ -50,9 +50,10 @@ __device__ __forceinline__ int pad_idx(int i, int sh) { return sh ? i + (i >> sh
 // padded (i + (i >> sh)); when a stride is a multiple of 2^sh the padded
 // positions of a butterfly are an arithmetic sequence and are stepped
 // instead of recomputed.
-template <class Rt, int RAD, int EPT>
+template <class Rt, int RAD, int EPT, bool FUSE, class LdF, class StF>
 __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl, int L, int Ns, FastDiv nsd,
-                                      const cx<Rt>* __restrict__ tw, int sh) {
+                                      const cx<Rt>* __restrict__ tw, int sh, bool gload, bool gstore, const LdF& ld,
+                                      const StF& st) {
   constexpr int NB = EPT / RAD;
   const int Lr = L / RAD;
   const int pmask = sh ? (1 << sh) - 1 : 0;
@@ -65,7 +66,10 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
   for (int b = 0; b < NB; ++b) {
     const int j = jl + b * Tl;
     if (active && j < Lr) {
-      if (rstep) {
+      if (FUSE && gload) {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) v[b][t] = ld(j + t * Lr);
+      } else if (rstep) {
         const int p0 = pad_idx(j, sh);
 #pragma unroll
         for (int t = 0; t < RAD; ++t) v[b][t] = line[p0 + t * rstride];
@@ -90,7 +94,10 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       const uint32_t q = fdiv((uint32_t)j, nsd);
       const uint32_t k = (uint32_t)j - q * (uint32_t)Ns;
       const int base = (int)(q * (uint32_t)Ns * RAD + k);
-      if (wstep) {
+      if (FUSE && gstore) {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) st(base + t * Ns, v[b][t]);
+      } else if (wstep) {
         const int p0 = pad_idx(base, sh);
 #pragma unroll
         for (int t = 0; t < RAD; ++t) line[p0 + t * wstride] = v[b][t];
@@ -103,25 +110,35 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
   __syncthreads();
 }
 
-template <class Rt, int EPT>
+// All radix stages of a line.  With fuse_first the first stage reads its
+// inputs through ld(element) from global memory instead of LDS; with
+// fuse_last the last stage writes through st(element, value).
+template <class Rt, int EPT, bool FUSE, class LdF, class StF>
 __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const LinePass& p,
-                                      const cx<Rt>* __restrict__ tw) {
+                                      const cx<Rt>* __restrict__ tw, bool fuse_first, bool fuse_last, const LdF& ld,
+                                      const StF& st) {
   int Ns = 1;
   const int sh = p.lds_sh;
-  for (int s = 0; s < p.nst; ++s) {
+  const int nst = p.nst;
+  for (int s = 0; s < nst; ++s) {
     const int r = p.rad[s];
     const FastDiv nsd = p.nsdiv[s];
+    const bool gl = fuse_first && s == 0;
+    const bool gs = fuse_last && s == nst - 1;
     switch (r) {
-      case 2: stage<Rt, 2, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 3: stage<Rt, 3, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 4: stage<Rt, 4, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 5: stage<Rt, 5, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 7: stage<Rt, 7, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 8: stage<Rt, 8, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 9: stage<Rt, 9, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 11: stage<Rt, 11, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 13: stage<Rt, 13, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
-      case 16: stage<Rt, 16, EPT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh); break;
+#define FFTAHIP_STAGE(R) \
+  case R: stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st); break;
+      FFTAHIP_STAGE(2)
+      FFTAHIP_STAGE(3)
+      FFTAHIP_STAGE(4)
+      FFTAHIP_STAGE(5)
+      FFTAHIP_STAGE(7)
+      FFTAHIP_STAGE(8)
+      FFTAHIP_STAGE(9)
+      FFTAHIP_STAGE(11)
+      FFTAHIP_STAGE(13)
+      FFTAHIP_STAGE(16)
+#undef FFTAHIP_STAGE
       default: break;
     }
     Ns *= r;
@@ -204,8 +221,12 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const int64_t twmask = (1ll << p.tw_sh) - 1;
   const bool conj_in = p.conj_in != 0;
 
+  const bool fuse_in = MODE == MODE_PLAIN && p.fuse_in != 0;
+  const bool fuse_out = MODE == MODE_PLAIN && p.fuse_out != 0;
+  const bool has_mid = MODE == MODE_FULL && p.mid_mode == MID_BHAT;
+
   // ---- load (global -> LDS), coalesced across lines or along a line
-  for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
+  for (int e0 = 0; e0 < (fuse_in ? 0 : total); e0 += BLOCK * U) {
     cx<Rt> v[U];
     int slot[U];
     int tt[U], cc[U];
@@ -260,14 +281,60 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   }
   __syncthreads();
 
-  // ---- radix sweeps in LDS
+  // ---- radix sweeps in LDS (first stage from / last stage to global memory
+  // when fused: the element reads/writes of line cl then go straight to HBM,
+  // coalesced across the threads of the line)
   const int cl = tid / p.Tl;
   const int jl = tid - cl * p.Tl;
   const bool active = (cl < nvalid) && (tid < C * p.Tl);
   cx<Rt>* myline = lds + (active ? cl : 0) * ldl;
-  sweep<Rt, EPT>(myline, jl, active, p, tw);
+  const int cs = active ? cl : 0;
+  const Rt scale = (Rt)p.scale;
+  const bool has_scale = p.scale != 1.0;
+  const bool conj_out = p.conj_out != 0;
+  auto ld = [&](int e) -> cx<Rt> {
+    cx<Rt> x = {(Rt)0, (Rt)0};
+    if (MODE == MODE_FULL && p.load_mode == LOAD_CHIRP) {
+      const int64_t n = sh.cin[cs] + (int64_t)e * p.cae;
+      if (n >= p.chirp_n) return x;
+      x = in[sh.in[cs] + (int64_t)e * p.ae];
+      if (conj_in) x = cconj(x);
+      x = cmul(x, cconj(reinterpret_cast<const cx<Rt>*>(p.chirp)[n]));
+    } else {
+      x = in[sh.in[cs] + (int64_t)e * p.ae];
+      if (conj_in) x = cconj(x);
+    }
+    if (MODE != MODE_PLAIN && p.pretw == 1) {
+      const int64_t m = sh.twk[cs] * (int64_t)e * p.tw_scale;
+      x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
+    }
+    return x;
+  };
+  auto st = [&](int e, cx<Rt> x) {
+    if (MODE == MODE_FULL) {
+      if (p.pretw == 2) {
+        const int64_t m = sh.twk[cs] * (int64_t)e * p.tw_scale;
+        x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
+      }
+      if (p.store_mode != STORE_PLAIN) {
+        const int64_t n = sh.cout[cs] + (int64_t)e * p.cbe;
+        if (p.store_mode == STORE_BHAT) {
+          x = cconj(cmul(x, reinterpret_cast<const cx<Rt>*>(p.bhat)[n]));
+        } else {
+          if (n >= p.chirp_n) return;
+          x = cconj(cmul(reinterpret_cast<const cx<Rt>*>(p.chirp)[n], x));
+        }
+      }
+    }
+    if (conj_out) x = cconj(x);
+    if (has_scale) x = cscale(x, scale);
+    out[sh.out[cs] + (int64_t)e * p.be] = x;
+  };
+  // only the plain variant carries the fused global paths (register cost)
+  constexpr bool kFuse = MODE == MODE_PLAIN;
+  sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, fuse_in, fuse_out && !has_mid, ld, st);
 
-  if (MODE == MODE_FULL && p.mid_mode == MID_BHAT) {
+  if (has_mid) {
     const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
     for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
       cx<Rt> f[U];
@@ -290,14 +357,11 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         if (slot[u] >= 0) lds[slot[u]] = cconj(cmul(lds[slot[u]], f[u]));
     }
     __syncthreads();
-    sweep<Rt, EPT>(myline, jl, active, p, tw);
+    sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, false, fuse_out, ld, st);
   }
 
   // ---- store (LDS -> global)
-  const Rt scale = (Rt)p.scale;
-  const bool has_scale = p.scale != 1.0;
-  const bool conj_out = p.conj_out != 0;
-  for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
+  for (int e0 = 0; e0 < (fuse_out ? 0 : total); e0 += BLOCK * U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * BLOCK + tid;

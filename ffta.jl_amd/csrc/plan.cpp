@@ -710,6 +710,13 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
     pad_cache[key] = {best_sh, ldl};
   }
   p.lds_sh = best_sh;
+  // row-mode sides go straight between HBM and the first/last radix stage
+  // when the threads of one line cover >= 128 contiguous bytes per access
+  // and the pass has no table-driven element operations
+  const bool plain = p.load_mode == LOAD_PLAIN && p.mid_mode == MID_NONE && p.store_mode == STORE_PLAIN && p.pretw == 0;
+  const bool fuse = env_i64("FFTAHIP_FUSE", 1) != 0 && !in.rad.empty() && plain && in.Tl * esz >= 128;
+  p.fuse_in = (fuse && !p.col_in) ? 1 : 0;
+  p.fuse_out = (fuse && !p.col_out) ? 1 : 0;
   const int64_t threads = C * in.Tl;
   p.C = (int32_t)C;
   p.ldl = (int32_t)ldl;
@@ -1838,7 +1845,7 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   os << ", \"L\": " << p.L << ", \"rad\": [";
   for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.rad[i];
   os << "], \"C\": " << p.C << ", \"Tl\": " << p.Tl << ", \"ldl\": " << p.ldl << ", \"lds_sh\": " << p.lds_sh << ", \"col_in\": " << p.col_in
-     << ", \"col_out\": " << p.col_out << ", \"nlines\": " << p.nlines << ", \"n\": [" << p.n[0] << ", " << p.n[1]
+     << ", \"col_out\": " << p.col_out << ", \"fuse_in\": " << p.fuse_in << ", \"fuse_out\": " << p.fuse_out << ", \"nlines\": " << p.nlines << ", \"n\": [" << p.n[0] << ", " << p.n[1]
      << ", " << p.n[2] << "], \"a\": [" << p.a[0] << ", " << p.a[1] << ", " << p.a[2] << ", " << p.a[3]
      << "], \"b\": [" << p.b[0] << ", " << p.b[1] << ", " << p.b[2] << ", " << p.b[3] << "], \"ae\": " << p.ae
      << ", \"be\": " << p.be << ", ";
