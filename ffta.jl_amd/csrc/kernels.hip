@@ -24,6 +24,43 @@ __device__ __forceinline__ cx<R> from_d2(double2 w) {
   return {(R)w.x, (R)w.y};
 }
 
+// Global-memory accesses.  Buffer and table pointers reach the kernels
+// through pass descriptors, so the compiler cannot prove that they point to
+// global memory and would emit flat instructions, which share their wait
+// counter with LDS traffic and so serialise table loads against the LDS
+// sweeps.  These helpers issue global instructions explicitly; a complex
+// value moves as one 2-vector (one dwordx2 / dwordx4 access per element).
+#define FFTAHIP_GLOBAL __attribute__((address_space(1)))
+template <class R>
+__device__ __forceinline__ cx<R> gld(const cx<R>* p) {
+  typedef R V __attribute__((ext_vector_type(2)));
+  const V v = *(const FFTAHIP_GLOBAL V*)p;
+  return {v.x, v.y};
+}
+__device__ __forceinline__ double2 gld(const double2* p) {
+  typedef double V __attribute__((ext_vector_type(2)));
+  const V v = *(const FFTAHIP_GLOBAL V*)p;
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ int64_t gld(const int64_t* p) { return *(const FFTAHIP_GLOBAL int64_t*)p; }
+template <class R>
+__device__ __forceinline__ void gst(cx<R>* p, cx<R> a) {
+  typedef R V __attribute__((ext_vector_type(2)));
+  V v;
+  v.x = a.x;
+  v.y = a.y;
+  *(FFTAHIP_GLOBAL V*)p = v;
+}
+
+// Conjugate when sgn = 0x80000000, identity when sgn = 0 (sign-bit flip of
+// the imaginary part, no branch).
+__device__ __forceinline__ cx<float> cconj_if(cx<float> a, uint32_t sgn) {
+  return {a.x, __uint_as_float(__float_as_uint(a.y) ^ sgn)};
+}
+__device__ __forceinline__ cx<double> cconj_if(cx<double> a, uint32_t sgn) {
+  return {a.x, __longlong_as_double(__double_as_longlong(a.y) ^ ((long long)sgn << 32))};
+}
+
 // exp(-2 pi i m / n) in FP64 for 0 <= m < n: reduce m to (-n/2, n/2] first so
 // that sincospi sees |x| <= 1.
 __device__ __forceinline__ double2 unit_root(int64_t m, int64_t n) {
@@ -66,6 +103,21 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
   for (int b = 0; b < NB; ++b) {
     const int j = jl + b * Tl;
     if (active && j < Lr) {
+      // FP32: the stage twiddles are requested first (global loads, L2
+      // latency) and consumed in the same branch region, after the LDS
+      // reads.  FP64 loads them as it consumes them (a full set in flight
+      // would not fit the register budget).
+      constexpr bool kPre = sizeof(Rt) == 4;
+      const cx<Rt>* __restrict__ w = tw;
+      cx<Rt> wv[RAD > 1 ? RAD - 1 : 1];
+      if (Ns > 1) {
+        const uint32_t k = (uint32_t)j - fdiv((uint32_t)j, nsd) * (uint32_t)Ns;
+        w = tw + (Ns - 1) + k * (RAD - 1);
+        if (kPre) {
+#pragma unroll
+          for (int t = 0; t < RAD - 1; ++t) wv[t] = gld(w + t);
+        }
+      }
       if (FUSE && gload) {
 #pragma unroll
         for (int t = 0; t < RAD; ++t) v[b][t] = ld(j + t * Lr);
@@ -78,10 +130,8 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
         for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, sh)];
       }
       if (Ns > 1) {
-        const uint32_t k = (uint32_t)j - fdiv((uint32_t)j, nsd) * (uint32_t)Ns;
-        const cx<Rt>* __restrict__ w = tw + (Ns - 1) + k * (RAD - 1) - 1;
 #pragma unroll
-        for (int t = 1; t < RAD; ++t) v[b][t] = cmul(v[b][t], w[t]);
+        for (int t = 1; t < RAD; ++t) v[b][t] = cmul(v[b][t], kPre ? wv[t - 1] : gld(w + t - 1));
       }
       dft_any<Rt, RAD>(v[b]);
     }
@@ -199,8 +249,8 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     idx[3] = q2;
     int64_t oi = idx[0] * p.a[0] + idx[1] * p.a[1] + idx[2] * p.a[2] + idx[3] * p.a[3];
     int64_t oo = idx[0] * p.b[0] + idx[1] * p.b[1] + idx[2] * p.b[2] + idx[3] * p.b[3];
-    if (p.in_tab) oi += p.in_tab[idx[3]];
-    if (p.out_tab) oo += p.out_tab[idx[3]];
+    if (p.in_tab) oi += gld(p.in_tab + idx[3]);
+    if (p.out_tab) oo += gld(p.out_tab + idx[3]);
     sh.in[c] = oi;
     sh.out[c] = oo;
     if (MODE != MODE_PLAIN) sh.twk[c] = idx[p.twdim & 3];
@@ -217,24 +267,34 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const cx<Rt>* __restrict__ in = reinterpret_cast<const cx<Rt>*>(inb) + p.in_off;
   cx<Rt>* __restrict__ out = reinterpret_cast<cx<Rt>*>(outb) + p.out_off;
   const cx<Rt>* __restrict__ tw = reinterpret_cast<const cx<Rt>*>(p.tw);
+  const cx<Rt>* __restrict__ chirp = reinterpret_cast<const cx<Rt>*>(p.chirp);
+  const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
   const int total = C * L;
   const int64_t twmask = (1ll << p.tw_sh) - 1;
-  const bool conj_in = p.conj_in != 0;
+  const uint32_t sgn_in = p.conj_in ? 0x80000000u : 0u;
+  const uint32_t sgn_out = p.conj_out ? 0x80000000u : 0u;
+  const bool chirp_in = MODE == MODE_FULL && p.load_mode == LOAD_CHIRP;
+  const bool pretw_in = MODE != MODE_PLAIN && p.pretw == 1;
+  const bool pretw_out = MODE == MODE_FULL && p.pretw == 2;
+  const int store_mode = MODE == MODE_FULL ? p.store_mode : STORE_PLAIN;
 
   const bool fuse_in = MODE == MODE_PLAIN && p.fuse_in != 0;
   const bool fuse_out = MODE == MODE_PLAIN && p.fuse_out != 0;
   const bool has_mid = MODE == MODE_FULL && p.mid_mode == MID_BHAT;
 
-  // ---- load (global -> LDS), coalesced across lines or along a line
+  // ---- load (global -> LDS), coalesced across lines or along a line.  All
+  // global loads of a batch (element, chirp factor, pass-twiddle halves) are
+  // issued before any of them is consumed.
   for (int e0 = 0; e0 < (fuse_in ? 0 : total); e0 += BLOCK * U) {
-    cx<Rt> v[U];
+    cx<Rt> v[U], f[U];
+    double2 wl[U], wh[U];
     int slot[U];
-    int tt[U], cc[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * BLOCK + tid;
       slot[u] = -1;
       v[u] = {(Rt)0, (Rt)0};
+      f[u] = {(Rt)0, (Rt)0};
       if (e < total) {
         int c, t;
         if (p.col_in) {
@@ -244,39 +304,30 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
           c = (int)fdiv((uint32_t)e, p.divL);
           t = e - c * L;
         }
-        tt[u] = t;
-        cc[u] = c;
         if (c < nvalid) {
           slot[u] = c * ldl + pad_idx(t, psh);
           bool live = true;
-          if (MODE == MODE_FULL && p.load_mode == LOAD_CHIRP) live = sh.cin[c] + (int64_t)t * p.cae < p.chirp_n;
-          if (live) v[u] = in[sh.in[c] + (int64_t)t * p.ae];
+          if (chirp_in) {
+            const int64_t n = sh.cin[c] + (int64_t)t * p.cae;
+            live = n < p.chirp_n;
+            if (live) f[u] = gld(chirp + n);
+          }
+          if (live) v[u] = gld(in + sh.in[c] + (int64_t)t * p.ae);
+          if (pretw_in) {
+            const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
+            wl[u] = gld(p.tw_lo + (m & twmask));
+            wh[u] = gld(p.tw_hi + (m >> p.tw_sh));
+          }
         }
       }
     }
-    if (MODE == MODE_PLAIN) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (slot[u] < 0) continue;
-        lds[slot[u]] = conj_in ? cconj(v[u]) : v[u];
-      }
-    } else {
-      // second batch: table factors (chirp, pass twiddle), consumed at once
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (slot[u] < 0) continue;
-        cx<Rt> x = conj_in ? cconj(v[u]) : v[u];
-        const int c = cc[u], t = tt[u];
-        if (MODE == MODE_FULL && p.load_mode == LOAD_CHIRP) {
-          const int64_t n = sh.cin[c] + (int64_t)t * p.cae;
-          if (n < p.chirp_n) x = cmul(x, cconj(reinterpret_cast<const cx<Rt>*>(p.chirp)[n]));
-        }
-        if (p.pretw == 1) {
-          const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
-          x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
-        }
-        lds[slot[u]] = x;
-      }
+    for (int u = 0; u < U; ++u) {
+      if (slot[u] < 0) continue;
+      cx<Rt> x = cconj_if(v[u], sgn_in);
+      if (chirp_in) x = cmul(x, cconj(f[u]));
+      if (pretw_in) x = cmul(x, from_d2<Rt>(dmul2(wl[u], wh[u])));
+      lds[slot[u]] = x;
     }
   }
   __syncthreads();
@@ -291,51 +342,44 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const int cs = active ? cl : 0;
   const Rt scale = (Rt)p.scale;
   const bool has_scale = p.scale != 1.0;
-  const bool conj_out = p.conj_out != 0;
   auto ld = [&](int e) -> cx<Rt> {
     cx<Rt> x = {(Rt)0, (Rt)0};
-    if (MODE == MODE_FULL && p.load_mode == LOAD_CHIRP) {
+    if (chirp_in) {
       const int64_t n = sh.cin[cs] + (int64_t)e * p.cae;
       if (n >= p.chirp_n) return x;
-      x = in[sh.in[cs] + (int64_t)e * p.ae];
-      if (conj_in) x = cconj(x);
-      x = cmul(x, cconj(reinterpret_cast<const cx<Rt>*>(p.chirp)[n]));
+      x = cmul(cconj_if(gld(in + sh.in[cs] + (int64_t)e * p.ae), sgn_in), cconj(gld(chirp + n)));
     } else {
-      x = in[sh.in[cs] + (int64_t)e * p.ae];
-      if (conj_in) x = cconj(x);
+      x = cconj_if(gld(in + sh.in[cs] + (int64_t)e * p.ae), sgn_in);
     }
-    if (MODE != MODE_PLAIN && p.pretw == 1) {
+    if (pretw_in) {
       const int64_t m = sh.twk[cs] * (int64_t)e * p.tw_scale;
-      x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
+      x = cmul(x, from_d2<Rt>(dmul2(gld(p.tw_lo + (m & twmask)), gld(p.tw_hi + (m >> p.tw_sh)))));
     }
     return x;
   };
   auto st = [&](int e, cx<Rt> x) {
-    if (MODE == MODE_FULL) {
-      if (p.pretw == 2) {
-        const int64_t m = sh.twk[cs] * (int64_t)e * p.tw_scale;
-        x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
-      }
-      if (p.store_mode != STORE_PLAIN) {
-        const int64_t n = sh.cout[cs] + (int64_t)e * p.cbe;
-        if (p.store_mode == STORE_BHAT) {
-          x = cconj(cmul(x, reinterpret_cast<const cx<Rt>*>(p.bhat)[n]));
-        } else {
-          if (n >= p.chirp_n) return;
-          x = cconj(cmul(reinterpret_cast<const cx<Rt>*>(p.chirp)[n], x));
-        }
+    if (pretw_out) {
+      const int64_t m = sh.twk[cs] * (int64_t)e * p.tw_scale;
+      x = cmul(x, from_d2<Rt>(dmul2(gld(p.tw_lo + (m & twmask)), gld(p.tw_hi + (m >> p.tw_sh)))));
+    }
+    if (store_mode != STORE_PLAIN) {
+      const int64_t n = sh.cout[cs] + (int64_t)e * p.cbe;
+      if (store_mode == STORE_BHAT) {
+        x = cconj(cmul(x, gld(bhat + n)));
+      } else {
+        if (n >= p.chirp_n) return;
+        x = cconj(cmul(gld(chirp + n), x));
       }
     }
-    if (conj_out) x = cconj(x);
+    x = cconj_if(x, sgn_out);
     if (has_scale) x = cscale(x, scale);
-    out[sh.out[cs] + (int64_t)e * p.be] = x;
+    gst(out + sh.out[cs] + (int64_t)e * p.be, x);
   };
   // only the plain variant carries the fused global paths (register cost)
   constexpr bool kFuse = MODE == MODE_PLAIN;
   sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, fuse_in, fuse_out && !has_mid, ld, st);
 
   if (has_mid) {
-    const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
     for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
       cx<Rt> f[U];
       int slot[U];
@@ -348,7 +392,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
           const int t = e - c * L;
           if (c < nvalid) {
             slot[u] = c * ldl + pad_idx(t, psh);
-            f[u] = bhat[sh.cmid[c] + (int64_t)t * p.cme];
+            f[u] = gld(bhat + sh.cmid[c] + (int64_t)t * p.cme);
           }
         }
       }
@@ -360,11 +404,39 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, false, fuse_out, ld, st);
   }
 
-  // ---- store (LDS -> global)
-  for (int e0 = 0; e0 < (fuse_out ? 0 : total); e0 += BLOCK * U) {
+  // ---- store (LDS -> global).  As on the load side, the table factors of a
+  // batch are all requested before the first store of the batch.
+  if (MODE == MODE_PLAIN) {
+    for (int e0 = 0; e0 < (fuse_out ? 0 : total); e0 += BLOCK * U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * BLOCK + tid;
+        if (e >= total) continue;
+        int c, t;
+        if (p.col_out) {
+          t = (int)fdiv((uint32_t)e, p.divC);
+          c = e - t * C;
+        } else {
+          c = (int)fdiv((uint32_t)e, p.divL);
+          t = e - c * L;
+        }
+        if (c >= nvalid) continue;
+        cx<Rt> y = cconj_if(lds[c * ldl + pad_idx(t, psh)], sgn_out);
+        if (has_scale) y = cscale(y, scale);
+        gst(out + sh.out[c] + (int64_t)t * p.be, y);
+      }
+    }
+    return;
+  }
+  for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
+    cx<Rt> x[U], f[U];
+    double2 wl[U], wh[U];
+    int64_t o[U];
+    bool ok[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * BLOCK + tid;
+      ok[u] = false;
       if (e >= total) continue;
       int c, t;
       if (p.col_out) {
@@ -375,25 +447,34 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         t = e - c * L;
       }
       if (c >= nvalid) continue;
-      cx<Rt> x = lds[c * ldl + pad_idx(t, psh)];
-      if (MODE == MODE_FULL) {
-        if (p.pretw == 2) {
-          const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
-          x = cmul(x, from_d2<Rt>(dmul2(p.tw_lo[m & twmask], p.tw_hi[m >> p.tw_sh])));
-        }
-        if (p.store_mode != STORE_PLAIN) {
-          const int64_t n = sh.cout[c] + (int64_t)t * p.cbe;
-          if (p.store_mode == STORE_BHAT) {
-            x = cconj(cmul(x, reinterpret_cast<const cx<Rt>*>(p.bhat)[n]));
-          } else {
-            if (n >= p.chirp_n) continue;
-            x = cconj(cmul(reinterpret_cast<const cx<Rt>*>(p.chirp)[n], x));
-          }
+      if (store_mode != STORE_PLAIN) {
+        const int64_t n = sh.cout[c] + (int64_t)t * p.cbe;
+        if (store_mode == STORE_BHAT) {
+          f[u] = gld(bhat + n);
+        } else {
+          if (n >= p.chirp_n) continue;
+          f[u] = gld(chirp + n);
         }
       }
-      if (conj_out) x = cconj(x);
-      if (has_scale) x = cscale(x, scale);
-      out[sh.out[c] + (int64_t)t * p.be] = x;
+      if (pretw_out) {
+        const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
+        wl[u] = gld(p.tw_lo + (m & twmask));
+        wh[u] = gld(p.tw_hi + (m >> p.tw_sh));
+      }
+      x[u] = lds[c * ldl + pad_idx(t, psh)];
+      o[u] = sh.out[c] + (int64_t)t * p.be;
+      ok[u] = true;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      cx<Rt> y = x[u];
+      if (pretw_out) y = cmul(y, from_d2<Rt>(dmul2(wl[u], wh[u])));
+      if (store_mode == STORE_BHAT) y = cconj(cmul(y, f[u]));
+      else if (store_mode == STORE_CHIRP) y = cconj(cmul(f[u], y));
+      y = cconj_if(y, sgn_out);
+      if (has_scale) y = cscale(y, scale);
+      gst(out + o[u], y);
     }
   }
 }
