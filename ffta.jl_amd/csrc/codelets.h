@@ -1,5 +1,6 @@
 // codelets.h -- forward in-register DFTs of radix 2, 3, 4, 5, 7, 8, 9, 11,
-// 13 and 16 (X[k] = sum_n x[n] exp(-2 pi i n k / R)).  The backward
+// 13 and 16 (X[k] = sum_n x[n] exp(-2 pi i n k / R)), and of every 13-smooth
+// composite length as a two-factor split of those.  The backward
 // transform is obtained by conjugating on load and on store, so only the
 // forward sign exists on the device.
 #pragma once
@@ -151,6 +152,15 @@ __device__ __forceinline__ void dft_split(cx<R>* v) {
   split_rows<R, P, Q>(v, y, std::make_integer_sequence<int, P>{});
 }
 
+// First factor of a composite length without a codelet of its own: the first
+// codelet radix of (16, 9, 8, 13, 11, 7, 5, 4, 3, 2) that divides N.
+constexpr int split_radix(int N) {
+  const int order[10] = {16, 9, 8, 13, 11, 7, 5, 4, 3, 2};
+  for (int r : order)
+    if (N % r == 0 && N > r) return r;
+  return N;
+}
+
 template <class R, int P>
 __device__ __forceinline__ void dft_any(cx<R>* v) {
   if constexpr (P == 1) {
@@ -166,8 +176,12 @@ __device__ __forceinline__ void dft_any(cx<R>* v) {
     dft_split<R, 3, 3>(v);
   } else if constexpr (P == 16) {
     dft_split<R, 4, 4>(v);
-  } else {
+  } else if constexpr (P == 5 || P == 7 || P == 11 || P == 13) {
     dft_prime<R, P>(v);
+  } else {
+    // 13-smooth composite (register line passes): two-factor split
+    static_assert(split_radix(P) < P, "no codelet split for this length");
+    dft_split<R, split_radix(P), P / split_radix(P)>(v);
   }
 }
 

@@ -7,6 +7,7 @@ namespace fftahip {
 
 constexpr int kMaxStages = 12;      // radix stages inside one in-LDS line FFT
 constexpr int kMaxLines = 256;      // lines per workgroup, at most
+constexpr int kRegBlock = 256;      // lines (threads) per workgroup of a register line pass
 // static LDS of a line-pass workgroup: six int64 per line
 constexpr int kLineSharedBytes = 6 * 8 * kMaxLines;
 

@@ -13,6 +13,11 @@ int pass_mode(const LinePass& p);
 hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
                                    int precision, int block, const BufPtrs& bp, int remap, size_t lds_bytes,
                                    int64_t grid, hipStream_t stream);
+// Register line passes (one short line per thread; lengths are the 13-smooth
+// L <= reg_pass_max(precision, mode)); tiles of kRegBlock lines.
+int reg_pass_max(int precision, int mode);
+hipError_t launch_reg_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
+                                  int precision, const BufPtrs& bp, int64_t grid, hipStream_t stream);
 hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t L, hipStream_t stream);
 hipError_t launch_gen_twiddle_2level(double2* lo, double2* hi, int64_t N, int sh, int64_t nhi, hipStream_t stream);
 hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_t N, hipStream_t stream);

@@ -224,6 +224,50 @@ __device__ __forceinline__ double2 dmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Offsets of line `line` of a pass: input/output element offsets, the index
+// the pass twiddle is taken over, and the chirp / spectrum positions.  The
+// line index is split over n[0], n[1], n[2] and a last (table) dimension.
+struct LineOffsets {
+  int64_t in, out, twk, cin, cout, cmid;
+};
+
+template <int MODE>
+__device__ __forceinline__ LineOffsets line_offsets(const LinePass& p, uint32_t line) {
+  int64_t idx[4];
+  const uint32_t q = fdiv(line, p.divn[0]);
+  idx[0] = line - q * p.divn[0].d;
+  const uint32_t q1 = fdiv(q, p.divn[1]);
+  idx[1] = q - q1 * p.divn[1].d;
+  const uint32_t q2 = fdiv(q1, p.divn[2]);
+  idx[2] = q1 - q2 * p.divn[2].d;
+  idx[3] = q2;
+  LineOffsets o;
+  o.in = idx[0] * p.a[0] + idx[1] * p.a[1] + idx[2] * p.a[2] + idx[3] * p.a[3];
+  o.out = idx[0] * p.b[0] + idx[1] * p.b[1] + idx[2] * p.b[2] + idx[3] * p.b[3];
+  if (p.in_tab) o.in += gld(p.in_tab + idx[3]);
+  if (p.out_tab) o.out += gld(p.out_tab + idx[3]);
+  o.twk = o.cin = o.cout = o.cmid = 0;
+  if (MODE != MODE_PLAIN) {
+    const int d = p.twdim & 3;
+    o.twk = d == 0 ? idx[0] : (d == 1 ? idx[1] : (d == 2 ? idx[2] : idx[3]));
+  }
+  if (MODE == MODE_FULL) {
+    o.cin = idx[0] * p.ca[0] + idx[1] * p.ca[1] + idx[2] * p.ca[2] + idx[3] * p.ca[3];
+    o.cout = idx[0] * p.cb[0] + idx[1] * p.cb[1] + idx[2] * p.cb[2] + idx[3] * p.cb[3];
+    o.cmid = idx[0] * p.cm[0] + idx[1] * p.cm[1] + idx[2] * p.cm[2] + idx[3] * p.cm[3];
+  }
+  return o;
+}
+
+// Pass twiddle w_M^m from the two-level FP64 tables (product in FP64, then
+// rounded): m = twk * t * tw_scale < M < 2^31.
+template <class Rt>
+__device__ __forceinline__ cx<Rt> pass_twiddle(const LinePass& p, int64_t twk, int t) {
+  const uint32_t m = (uint32_t)twk * (uint32_t)(t * p.tw_scale);
+  const uint32_t mask = (1u << p.tw_sh) - 1u;
+  return from_d2<Rt>(dmul2(gld(p.tw_lo + (m & mask)), gld(p.tw_hi + (m >> p.tw_sh))));
+}
+
 template <class Rt, int EPT, int BLOCK, int MODE>
 __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& bp, uint32_t tile, cx<Rt>* lds,
                                               LineShared& sh) {
@@ -238,26 +282,14 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const int nvalid = rem < C ? (int)rem : C;
 
   for (int c = tid; c < nvalid; c += BLOCK) {
-    const uint32_t line = (uint32_t)(line0 + c);
-    int64_t idx[4];
-    uint32_t q = fdiv(line, p.divn[0]);
-    idx[0] = line - q * p.divn[0].d;
-    uint32_t q1 = fdiv(q, p.divn[1]);
-    idx[1] = q - q1 * p.divn[1].d;
-    uint32_t q2 = fdiv(q1, p.divn[2]);
-    idx[2] = q1 - q2 * p.divn[2].d;
-    idx[3] = q2;
-    int64_t oi = idx[0] * p.a[0] + idx[1] * p.a[1] + idx[2] * p.a[2] + idx[3] * p.a[3];
-    int64_t oo = idx[0] * p.b[0] + idx[1] * p.b[1] + idx[2] * p.b[2] + idx[3] * p.b[3];
-    if (p.in_tab) oi += gld(p.in_tab + idx[3]);
-    if (p.out_tab) oo += gld(p.out_tab + idx[3]);
-    sh.in[c] = oi;
-    sh.out[c] = oo;
-    if (MODE != MODE_PLAIN) sh.twk[c] = idx[p.twdim & 3];
+    const LineOffsets o = line_offsets<MODE>(p, (uint32_t)(line0 + c));
+    sh.in[c] = o.in;
+    sh.out[c] = o.out;
+    if (MODE != MODE_PLAIN) sh.twk[c] = o.twk;
     if (MODE == MODE_FULL) {
-      sh.cin[c] = idx[0] * p.ca[0] + idx[1] * p.ca[1] + idx[2] * p.ca[2] + idx[3] * p.ca[3];
-      sh.cout[c] = idx[0] * p.cb[0] + idx[1] * p.cb[1] + idx[2] * p.cb[2] + idx[3] * p.cb[3];
-      sh.cmid[c] = idx[0] * p.cm[0] + idx[1] * p.cm[1] + idx[2] * p.cm[2] + idx[3] * p.cm[3];
+      sh.cin[c] = o.cin;
+      sh.cout[c] = o.cout;
+      sh.cmid[c] = o.cmid;
     }
   }
   __syncthreads();
@@ -270,7 +302,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const cx<Rt>* __restrict__ chirp = reinterpret_cast<const cx<Rt>*>(p.chirp);
   const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
   const int total = C * L;
-  const int64_t twmask = (1ll << p.tw_sh) - 1;
+  const uint32_t twmask = (1u << p.tw_sh) - 1u;  // pass-twiddle exponents m < M < 2^31
   const uint32_t sgn_in = p.conj_in ? 0x80000000u : 0u;
   const uint32_t sgn_out = p.conj_out ? 0x80000000u : 0u;
   const bool chirp_in = MODE == MODE_FULL && p.load_mode == LOAD_CHIRP;
@@ -314,7 +346,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
           }
           if (live) v[u] = gld(in + sh.in[c] + (int64_t)t * p.ae);
           if (pretw_in) {
-            const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
+            const uint32_t m = (uint32_t)sh.twk[c] * (uint32_t)(t * p.tw_scale);
             wl[u] = gld(p.tw_lo + (m & twmask));
             wh[u] = gld(p.tw_hi + (m >> p.tw_sh));
           }
@@ -352,14 +384,14 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
       x = cconj_if(gld(in + sh.in[cs] + (int64_t)e * p.ae), sgn_in);
     }
     if (pretw_in) {
-      const int64_t m = sh.twk[cs] * (int64_t)e * p.tw_scale;
+      const uint32_t m = (uint32_t)sh.twk[cs] * (uint32_t)(e * p.tw_scale);
       x = cmul(x, from_d2<Rt>(dmul2(gld(p.tw_lo + (m & twmask)), gld(p.tw_hi + (m >> p.tw_sh)))));
     }
     return x;
   };
   auto st = [&](int e, cx<Rt> x) {
     if (pretw_out) {
-      const int64_t m = sh.twk[cs] * (int64_t)e * p.tw_scale;
+      const uint32_t m = (uint32_t)sh.twk[cs] * (uint32_t)(e * p.tw_scale);
       x = cmul(x, from_d2<Rt>(dmul2(gld(p.tw_lo + (m & twmask)), gld(p.tw_hi + (m >> p.tw_sh)))));
     }
     if (store_mode != STORE_PLAIN) {
@@ -457,7 +489,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         }
       }
       if (pretw_out) {
-        const int64_t m = sh.twk[c] * (int64_t)t * p.tw_scale;
+        const uint32_t m = (uint32_t)sh.twk[c] * (uint32_t)(t * p.tw_scale);
         wl[u] = gld(p.tw_lo + (m & twmask));
         wh[u] = gld(p.tw_hi + (m >> p.tw_sh));
       }
@@ -503,6 +535,126 @@ line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __re
   }
   const LinePass& p = passes[lo];
   run_line_pass<Rt, EPT, BLOCK, MODE>(p, bp, bid - tile_start[lo], reinterpret_cast<cx<Rt>*>(smem_raw), sh);
+}
+
+// ------------------------------------------------------------------ register line passes
+
+// A pass of short lines laid out across memory (line c+1 starts one element
+// after line c on both sides) runs one line per thread, entirely in
+// registers: each of the L loads and L stores of a wave covers 64
+// consecutive elements, and there is no LDS staging and no barrier.  The
+// line FFT is one compile-time codelet (dft_any<L>).  Same pass semantics as
+// run_line_pass except the mid-pass spectrum product (not supported here).
+constexpr int kRegChunk = 4;
+
+template <class Rt, int L, int MODE>
+__device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, uint32_t line) {
+  const LineOffsets o = line_offsets<MODE>(p, line);
+  const void* inb = p.in_buf == BUF_IN ? bp.in : (p.in_buf == BUF_OUT ? (const void*)bp.out : (const void*)bp.scratch);
+  void* outb = p.out_buf == BUF_OUT ? bp.out : bp.scratch;
+  const cx<Rt>* __restrict__ in = reinterpret_cast<const cx<Rt>*>(inb) + p.in_off + o.in;
+  cx<Rt>* __restrict__ out = reinterpret_cast<cx<Rt>*>(outb) + p.out_off + o.out;
+  const cx<Rt>* __restrict__ chirp = reinterpret_cast<const cx<Rt>*>(p.chirp);
+  const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
+  const uint32_t sgn_in = p.conj_in ? 0x80000000u : 0u;
+  const uint32_t sgn_out = p.conj_out ? 0x80000000u : 0u;
+  const bool chirp_in = MODE == MODE_FULL && p.load_mode == LOAD_CHIRP;
+  const bool pretw_in = MODE != MODE_PLAIN && p.pretw == 1;
+  const bool pretw_out = MODE == MODE_FULL && p.pretw == 2;
+  const int store_mode = MODE == MODE_FULL ? p.store_mode : STORE_PLAIN;
+
+  cx<Rt> v[L];
+  // positions past the chirp length read the line's first element (always
+  // in bounds) and are zeroed, so the loads stay unconditional
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    if (chirp_in) {
+      const int64_t n = o.cin + (int64_t)t * p.cae;
+      const bool live = n < p.chirp_n;
+      const cx<Rt> x = gld(in + (live ? (int64_t)t * p.ae : 0));
+      const cx<Rt> c = gld(chirp + (live ? n : 0));
+      const cx<Rt> y = cmul(cconj_if(x, sgn_in), cconj(c));
+      v[t] = live ? y : cx<Rt>{(Rt)0, (Rt)0};
+      // keep at most kRegChunk table loads in flight (register budget)
+      if (t % kRegChunk == kRegChunk - 1) __builtin_amdgcn_sched_barrier(0);
+    } else {
+      v[t] = cconj_if(gld(in + (int64_t)t * p.ae), sgn_in);
+    }
+  }
+  if (pretw_in) {
+#pragma unroll
+    for (int t = 1; t < L; ++t) {
+      v[t] = cmul(v[t], pass_twiddle<Rt>(p, o.twk, t));
+      if (t % kRegChunk == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  dft_any<Rt, L>(v);
+  const Rt scale = (Rt)p.scale;
+  const bool has_scale = p.scale != 1.0;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    cx<Rt> y = v[k];
+    if (pretw_out && k > 0) y = cmul(y, pass_twiddle<Rt>(p, o.twk, k));
+    bool live = true;
+    if (store_mode == STORE_BHAT) {
+      y = cconj(cmul(y, gld(bhat + o.cout + (int64_t)k * p.cbe)));
+    } else if (store_mode == STORE_CHIRP) {
+      const int64_t n = o.cout + (int64_t)k * p.cbe;
+      live = n < p.chirp_n;
+      y = cconj(cmul(gld(chirp + (live ? n : 0)), y));
+    }
+    y = cconj_if(y, sgn_out);
+    if (has_scale) y = cscale(y, scale);
+    if (live) gst(out + (int64_t)k * p.be, y);
+    if (MODE != MODE_PLAIN && k % kRegChunk == kRegChunk - 1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Line lengths of the register passes: every 13-smooth length up to
+// reg_max (one line's values, and the table factors in flight, must fit the
+// 128 registers): plain passes FP32 48 / FP64 24, load-side pass twiddle
+// FP32 32 / FP64 16, the full set of fused operations FP32 24 / FP64 12.
+template <class Rt, int MODE>
+constexpr int reg_max() {
+  constexpr int f32 = MODE == MODE_PLAIN ? 48 : (MODE == MODE_PRETW ? 32 : 24);
+  return sizeof(Rt) == 4 ? f32 : f32 / 2;
+}
+
+template <class Rt, int MODE>
+__device__ __forceinline__ void reg_dispatch(const LinePass& p, const BufPtrs& bp, uint32_t line) {
+  switch (p.L) {
+#define FFTAHIP_REG(N) \
+  case N:              \
+    if constexpr (N <= reg_max<Rt, MODE>()) reg_line<Rt, N, MODE>(p, bp, line); \
+    break;
+    FFTAHIP_REG(2) FFTAHIP_REG(3) FFTAHIP_REG(4) FFTAHIP_REG(5) FFTAHIP_REG(6) FFTAHIP_REG(7) FFTAHIP_REG(8)
+    FFTAHIP_REG(9) FFTAHIP_REG(10) FFTAHIP_REG(11) FFTAHIP_REG(12) FFTAHIP_REG(13) FFTAHIP_REG(14) FFTAHIP_REG(15)
+    FFTAHIP_REG(16) FFTAHIP_REG(18) FFTAHIP_REG(20) FFTAHIP_REG(21) FFTAHIP_REG(22) FFTAHIP_REG(24) FFTAHIP_REG(25)
+    FFTAHIP_REG(26) FFTAHIP_REG(27) FFTAHIP_REG(28) FFTAHIP_REG(30) FFTAHIP_REG(32) FFTAHIP_REG(33) FFTAHIP_REG(35)
+    FFTAHIP_REG(36) FFTAHIP_REG(39) FFTAHIP_REG(40) FFTAHIP_REG(42) FFTAHIP_REG(44) FFTAHIP_REG(45) FFTAHIP_REG(48)
+#undef FFTAHIP_REG
+    default: break;
+  }
+}
+
+// Work-list launch of register passes: tile = 256 consecutive lines of one
+// group (pass); tiles in the XCD-aware order of line_pass_group_kernel.
+template <class Rt, int MODE>
+__global__ void __launch_bounds__(kRegBlock) __attribute__((amdgpu_waves_per_eu(4)))
+reg_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __restrict__ tile_start, int ngroups,
+                      BufPtrs bp, uint32_t ntiles) {
+  const uint32_t bid = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (bid >= ntiles) return;
+  int lo = 0, hi = ngroups - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_start[mid] <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  const LinePass& p = passes[lo];
+  const int64_t line = (int64_t)(bid - tile_start[lo]) * kRegBlock + threadIdx.x;
+  if (line >= p.nlines) return;
+  reg_dispatch<Rt, MODE>(p, bp, (uint32_t)line);
 }
 
 // ------------------------------------------------------------------ tables
@@ -644,6 +796,37 @@ hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_t
   if (precision == 32)
     return launch_grp_block<float>(d_passes, d_tile_start, ngroups, block, mode, bp, remap, lds_bytes, grid, stream);
   return launch_grp_block<double>(d_passes, d_tile_start, ngroups, block, mode, bp, remap, lds_bytes, grid, stream);
+}
+
+namespace {
+template <class Rt, int MODE>
+hipError_t launch_reg(const LinePass* P, const uint32_t* T, int ng, const BufPtrs& bp, int64_t grid, hipStream_t s) {
+  const uint32_t padded = (uint32_t)((grid + 7) & ~7ll);
+  hipLaunchKernelGGL((reg_pass_group_kernel<Rt, MODE>), dim3(padded), dim3(kRegBlock), 0, s, P, T, ng, bp,
+                     (uint32_t)grid);
+  return hipGetLastError();
+}
+template <class Rt>
+hipError_t launch_reg_mode(const LinePass* P, const uint32_t* T, int ng, int mode, const BufPtrs& bp, int64_t grid,
+                           hipStream_t s) {
+  if (mode == MODE_PLAIN) return launch_reg<Rt, MODE_PLAIN>(P, T, ng, bp, grid, s);
+  if (mode == MODE_PRETW) return launch_reg<Rt, MODE_PRETW>(P, T, ng, bp, grid, s);
+  return launch_reg<Rt, MODE_FULL>(P, T, ng, bp, grid, s);
+}
+}  // namespace
+
+int reg_pass_max(int precision, int mode) {
+  if (precision == 32)
+    return mode == MODE_PLAIN ? reg_max<float, MODE_PLAIN>()
+                              : (mode == MODE_PRETW ? reg_max<float, MODE_PRETW>() : reg_max<float, MODE_FULL>());
+  return mode == MODE_PLAIN ? reg_max<double, MODE_PLAIN>()
+                            : (mode == MODE_PRETW ? reg_max<double, MODE_PRETW>() : reg_max<double, MODE_FULL>());
+}
+
+hipError_t launch_reg_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
+                                  int precision, const BufPtrs& bp, int64_t grid, hipStream_t stream) {
+  if (precision == 32) return launch_reg_mode<float>(d_passes, d_tile_start, ngroups, mode, bp, grid, stream);
+  return launch_reg_mode<double>(d_passes, d_tile_start, ngroups, mode, bp, grid, stream);
 }
 
 hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t L, hipStream_t stream) {

@@ -395,10 +395,21 @@ static int64_t row_max(int prec) {
 // Bluestein length M = M1 * M2 for the three fused passes: column passes of
 // length M1 (strided, <= pass_max) and one row pass of length M2 (<= row_max).
 static bool blue3_split(int64_t M, int prec, int64_t* M1, int64_t* M2) {
-  // prefer column passes of >= 32 points (enough work per strided tile),
-  // then the longest row that fits
   const int64_t pmax = pass_max(prec);
   Inner a, b;
+  // first choice: the longest columns that run as register line passes
+  const int64_t rmax = env_i64("FFTAHIP_REG", 1) != 0 ? reg_pass_max(prec, 2) : 0;
+  for (int64_t m1 = rmax; m1 >= 8; --m1) {
+    if (M % m1) continue;
+    const int64_t m2 = M / m1;
+    if (m2 > row_max(prec)) break;
+    if (!make_inner(m2, b) || !make_inner(m1, a)) continue;
+    *M1 = m1;
+    *M2 = m2;
+    return true;
+  }
+  // else LDS column passes of >= 32 points (enough work per strided tile),
+  // then the longest row that fits
   for (int64_t min_m1 : {(int64_t)32, (int64_t)2}) {
     for (int64_t m2 = std::min(M / min_m1, row_max(prec)); m2 >= 2; --m2) {
       if (M % m2) continue;
@@ -535,6 +546,7 @@ struct Dim {
 enum LaunchOp { OP_PASS = 0, OP_COPY = 1 };
 struct Launch {
   int op = OP_PASS;  // OP_COPY: copy `bytes` from the input buffer to the output buffer
+  bool reg = false;  // register line pass (one line per thread) instead of the LDS line pass
   LinePass lp;
   int block = 256;
   size_t lds = 0;
@@ -640,6 +652,13 @@ static double lds_cost(const Inner& in, int C, int ldl, int sh, int words, bool 
   return cost;
 }
 
+static bool reg_eligible(const LinePass& p, const Inner& in, int prec) {
+  if (env_i64("FFTAHIP_REG", 1) == 0) return false;
+  if (p.mid_mode != MID_NONE || !p.col_in || !p.col_out) return false;
+  if (in.L < 2 || in.L > reg_pass_max(prec, pass_mode(p))) return false;
+  return p.n[0] * (int64_t)csize(prec) >= 128;
+}
+
 static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   p.L = (int32_t)in.L;
   p.nst = (int32_t)in.rad.size();
@@ -726,6 +745,17 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   L.lds = (size_t)(C * ldl * esz);
   L.grid = (p.nlines + C - 1) / C;
   p.xcd_remap = (L.grid >= 16 && L.grid % 8 == 0) ? 1 : 0;
+  // short lines laid across memory on both sides (>= 128 contiguous bytes
+  // per wave access): one line per thread in registers
+  if (reg_eligible(p, in, prec)) {
+    L.reg = true;
+    L.block = kRegBlock;
+    L.lds = 0;
+    L.grid = (p.nlines + kRegBlock - 1) / kRegBlock;
+    p.C = kRegBlock;
+    p.fuse_in = p.fuse_out = 0;
+    p.xcd_remap = 0;
+  }
 }
 
 static LinePass blank_pass() {
@@ -1013,6 +1043,11 @@ static fftahip_status run_schedule(const DevSchedule& S, int prec, const BufPtrs
     }
     if (la.grid <= 0) continue;
     if (la.grid > 0x7fffffffLL) return fail(FFTAHIP_ERR_CAPACITY, "grid too large");
+    if (la.reg) {
+      HIP_TRY(launch_reg_pass_groups((const LinePass*)S.passes.p + i, (const uint32_t*)S.zero.p, 1, pass_mode(la.lp),
+                                     prec, bp, la.grid, s));
+      continue;
+    }
     HIP_TRY(launch_line_pass_groups((const LinePass*)S.passes.p + i, (const uint32_t*)S.zero.p, 1, pass_mode(la.lp),
                                     prec, la.block, bp, la.lp.xcd_remap, la.lds, la.grid, s));
   }
@@ -1141,6 +1176,9 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
   } else if (!lp.blue) {
     lp.M = N;
   }
+  // line indices, in-line positions and pass-twiddle exponents are 32-bit
+  if (lp.M >= (int64_t(1) << 31))
+    return fail(FFTAHIP_ERR_CAPACITY, "transform length " + std::to_string(N) + " needs more than 2^31 - 1 points");
   return FFTAHIP_SUCCESS;
 }
 
@@ -1198,6 +1236,7 @@ struct fftahip_batch_s {
   DevBuf passes;   // device LinePass arrays per launch
   DevBuf tiles;
   struct Phase {
+    bool reg;  // register line passes (block kRegBlock, no LDS)
     int block;
     int mode;
     size_t lds;
@@ -1672,10 +1711,33 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
   size_t maxp = 0;
   for (auto& L : per_group) maxp = std::max(maxp, L.size());
   for (size_t ph = 0; ph < maxp; ++ph) {
+    // register passes of this phase: one launch per mode
+    for (int mode = 0; mode < 3; ++mode) {
+      fftahip_batch_s::Phase P;
+      P.reg = true;
+      P.block = kRegBlock;
+      P.mode = mode;
+      P.lds = 0;
+      P.ngroups = 0;
+      P.pass_off = b->host_passes.size();
+      P.tile_off = b->host_tiles.size();
+      uint32_t tiles = 0;
+      for (auto& L : per_group) {
+        if (ph >= L.size() || !L[ph].reg || L[ph].grid <= 0 || pass_mode(L[ph].lp) != mode) continue;
+        b->host_passes.push_back(L[ph].lp);
+        b->host_tiles.push_back(tiles);
+        tiles += (uint32_t)L[ph].grid;
+        ++P.ngroups;
+      }
+      if (P.ngroups == 0) continue;
+      P.grid = tiles;
+      b->phases.push_back(P);
+    }
     for (int mode = 0; mode < 3; ++mode)
     for (int block : {256, 512, 1024}) {
       for (size_t cls : kLdsClass) {
         fftahip_batch_s::Phase P;
+        P.reg = false;
         P.block = block;
         P.mode = mode;
         P.lds = 0;
@@ -1684,8 +1746,8 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
         P.tile_off = b->host_tiles.size();
         uint32_t tiles = 0;
         for (auto& L : per_group) {
-          if (ph >= L.size() || L[ph].block != block || L[ph].grid <= 0 || lds_class(L[ph].lds) != cls ||
-              pass_mode(L[ph].lp) != mode)
+          if (ph >= L.size() || L[ph].reg || L[ph].block != block || L[ph].grid <= 0 ||
+              lds_class(L[ph].lds) != cls || pass_mode(L[ph].lp) != mode)
             continue;
           LinePass lp = L[ph].lp;
           lp.xcd_remap = 0;
@@ -1717,6 +1779,11 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
 static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
   const BufPtrs bp{d_in, d_out, b->scratch_ptr};
   for (auto& P : b->phases) {
+    if (P.reg) {
+      HIP_TRY(launch_reg_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
+                                     P.ngroups, P.mode, b->prec, bp, P.grid, s));
+      continue;
+    }
     HIP_TRY(launch_line_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
                                     P.ngroups, P.mode, b->prec, P.block, bp, 0, P.lds, P.grid, s));
   }
@@ -1767,7 +1834,7 @@ fftahip_status fftahip_batch_describe(fftahip_batch b, char* buf, size_t cap, si
      << ", \"lengths\": " << b->lens.size() << ", \"phases\": [";
   for (size_t i = 0; i < b->phases.size(); ++i) {
     auto& P = b->phases[i];
-    os << (i ? ", " : "") << "{\"block\": " << P.block << ", \"lds\": " << P.lds << ", \"grid\": " << P.grid
+    os << (i ? ", " : "") << "{\"reg\": " << (P.reg ? 1 : 0) << ", \"mode\": " << P.mode << ", \"block\": " << P.block << ", \"lds\": " << P.lds << ", \"grid\": " << P.grid
        << ", \"groups\": " << P.ngroups << "}";
   }
   os << "], \"scratch_bytes\": " << b->scratch.bytes << "}";
@@ -1837,7 +1904,7 @@ static void json_buf(std::ostringstream& os, const char* name, int id, int64_t o
 
 static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   const LinePass& p = la.lp;
-  os << "{\"op\": " << la.op << ", \"block\": " << la.block << ", \"lds\": " << la.lds << ", \"grid\": " << la.grid
+  os << "{\"op\": " << la.op << ", \"reg\": " << (la.reg ? 1 : 0) << ", \"block\": " << la.block << ", \"lds\": " << la.lds << ", \"grid\": " << la.grid
      << ", \"bytes\": " << la.bytes << ", ";
   json_buf(os, "in", la.op == OP_COPY ? BUF_IN : p.in_buf, la.op == OP_COPY ? 0 : p.in_off, prec);
   os << ", ";

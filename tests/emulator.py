@@ -17,6 +17,10 @@ import numpy as np
 TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
 EPT = 16
 STATIC_LDS = 6 * 8 * 256
+# register line passes: lines per workgroup, and the lengths with a codelet
+REG_BLOCK = 256
+REG_LENGTHS = {n for n in range(2, 49) if all(q in (2, 3, 5, 7, 11, 13) for q in
+               [f for f in range(2, n + 1) if n % f == 0 and all(f % d for d in range(2, f))])}
 
 
 class ScheduleError(AssertionError):
@@ -150,22 +154,33 @@ class Emulator:
 
     def _pass(self, p, lds_per_block):
         L, C, Tl = p["L"], p["C"], p["Tl"]
-        if C < 1 or C > 256 or C * Tl > p["block"]:
-            raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
-        sh = p["lds_sh"]
-        need = L + ((L - 1) >> sh if sh else 0)
-        if p["ldl"] < need or p["lds"] != C * p["ldl"] * self.esize:
-            raise ScheduleError("LDS size mismatch")
-        if p["lds"] + STATIC_LDS > lds_per_block:
-            raise ScheduleError(f"LDS {p['lds']} + static exceeds {lds_per_block}")
-        if p["grid"] * C < p["nlines"]:
-            raise ScheduleError("grid does not cover all lines")
-        if p["xcd_remap"] and p["grid"] % 8:
-            raise ScheduleError("XCD remap needs grid % 8 == 0")
-        key = (L, tuple(p["rad"]), Tl, p["lds_sh"])
-        if key not in self.sweeps_checked:
-            emulate_sweep(L, p["rad"], Tl, p["lds_sh"])
-            self.sweeps_checked.add(key)
+        if p.get("reg"):
+            # register line pass: one line per thread, blocks of REG_BLOCK lines
+            mode = 2 if (p["load_mode"] or p["mid_mode"] or p["store_mode"] or p["pretw"] == 2) else int(p["pretw"] == 1)
+            lmax = {0: 48, 1: 32, 2: 24}[mode] // (1 if self.esize == 8 else 2)
+            if L not in REG_LENGTHS or L > lmax:
+                raise ScheduleError(f"register pass of unsupported length {L}")
+            if p["block"] != REG_BLOCK or p["lds"] != 0 or p["grid"] * REG_BLOCK < p["nlines"]:
+                raise ScheduleError("register pass geometry")
+            if p["mid_mode"] or not (p["col_in"] and p["col_out"]):
+                raise ScheduleError("register pass needs column layout on both sides and no mid product")
+        else:
+            if C < 1 or C > 256 or C * Tl > p["block"]:
+                raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
+            sh = p["lds_sh"]
+            need = L + ((L - 1) >> sh if sh else 0)
+            if p["ldl"] < need or p["lds"] != C * p["ldl"] * self.esize:
+                raise ScheduleError("LDS size mismatch")
+            if p["lds"] + STATIC_LDS > lds_per_block:
+                raise ScheduleError(f"LDS {p['lds']} + static exceeds {lds_per_block}")
+            if p["grid"] * C < p["nlines"]:
+                raise ScheduleError("grid does not cover all lines")
+            if p["xcd_remap"] and p["grid"] % 8:
+                raise ScheduleError("XCD remap needs grid % 8 == 0")
+            key = (L, tuple(p["rad"]), Tl, p["lds_sh"])
+            if key not in self.sweeps_checked:
+                emulate_sweep(L, p["rad"], Tl, p["lds_sh"])
+                self.sweeps_checked.add(key)
         if p["tw"] is None or p["tw"][0] != TK_STAGE or p["tw"][1] != L:
             raise ScheduleError("stage twiddle table does not match L")
         nl = p["nlines"]
@@ -212,6 +227,11 @@ class Emulator:
         sel = addr_in[valid_in]
         if sel.size and (sel.min() < 0 or sel.max() >= src.size):
             raise ScheduleError(f"load out of bounds [{sel.min()}, {sel.max()}] of {src.size}")
+        if p.get("reg") and p["load_mode"] == 1:
+            # dead chirp positions read the line's first element instead
+            first = addr_in[:, 0]
+            if first.min() < 0 or first.max() >= src.size:
+                raise ScheduleError("register pass: first element of a line out of bounds")
         v = np.zeros(addr_in.shape, dtype=np.complex128)
         v[valid_in] = src[sel]
         if np.isnan(v[valid_in]).any():

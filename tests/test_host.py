@@ -199,3 +199,14 @@ def test_schedule_batch_uneven():
         ref = np.fft.fft(xi) if d == -1 else np.fft.ifft(xi) * n
         assert np.linalg.norm(y[off:off + n] - ref) <= 1e-12 * max(np.linalg.norm(ref), 1e-300)
         off += n
+
+
+@pytest.mark.parametrize("n,prec", [(20011, 32), (3199, 32), (28693, 32), (65521, 32), (10007, 64), (4036, 64),
+                                    (3 * 4099, 32)])
+def test_register_passes(n, prec):
+    """Short column passes run as register line passes (one line per thread);
+    the emulator checks their geometry and result like any other pass."""
+    s = P.debug_schedule((n,), prec, -1)
+    assert any(la.get("reg") for la in s["launches"]), [(la["L"], la["col_in"], la["col_out"]) for la in s["launches"]]
+    _check_schedule((n,), prec, -1)
+    _check_schedule((n,), prec, 1)
