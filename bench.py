@@ -18,6 +18,15 @@ Reported (one JSON line):
   cpu_baseline  the oracle (C++ restatement of FFTA's algorithm, FP32 like
              FFTA's ComplexF32 path) timed on a sample of the same workload,
              one transform per thread.
+  serving    Poisson replay through the native request loop: p50/p99 latency,
+             throughput.
+  percall_fp64  the same library called once per transform in ComplexF64 on
+             a sample of the workload (transforms/s), and the batched FP32
+             rate over it.
+  per_call_configs  single transforms (4096 CF64 fft/bfft; 177147,
+             31104000, 1000003; 4096x4096, 3000x2187; CF32 and CF64): GPU
+             time per mul!, GFLOP/s, fraction of HBM peak, error against the
+             FP64 oracle, and the oracle timed on one host core.
 """
 from __future__ import annotations
 
@@ -200,6 +209,148 @@ def parity_sample(torch, batch, x, y, lengths, dirs, offsets, k: int = 200, seed
     return worst, len(idx)
 
 
+PER_CALL_CONFIGS = [
+    # (name, Julia shape, precision, direction, input distribution)
+    ("fft 4096", (4096,), 64, -1, "uniform"),
+    ("bfft 4096", (4096,), 64, 1, "uniform"),
+    ("fft 177147", (177147,), 64, -1, "normal"),
+    ("fft 177147", (177147,), 32, -1, "normal"),
+    ("fft 31104000", (31104000,), 64, -1, "normal"),
+    ("fft 31104000", (31104000,), 32, -1, "normal"),
+    ("fft 1000003", (1000003,), 64, -1, "normal"),
+    ("fft 1000003", (1000003,), 32, -1, "normal"),
+    ("fft 4096x4096", (4096, 4096), 64, -1, "normal"),
+    ("fft 4096x4096", (4096, 4096), 32, -1, "normal"),
+    ("fft 3000x2187", (3000, 2187), 64, -1, "normal"),
+    ("fft 3000x2187", (3000, 2187), 32, -1, "normal"),
+]
+
+
+def per_call_configs(torch, cpu: bool = True, seed: int = 7, cpu_budget_s: float = 3.0):
+    """The single-transform configurations: GPU time per mul! with the arrays
+    resident in HBM (HIP events on the launch stream), error against the FP64
+    oracle, and the oracle in the plan's precision timed single-threaded on
+    the host (FFTA runs one transform on one core)."""
+    import oracle
+    from fftahip.plan import Plan
+
+    rows = []
+    stream = torch.cuda.current_stream()
+    for name, shape, prec, direction, dist in PER_CALL_CONFIGS:
+        rng = np.random.default_rng(seed)
+        n = int(np.prod(shape))
+        if dist == "uniform":
+            x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        else:
+            x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        dt = np.complex64 if prec == 32 else np.complex128
+        tdt = torch.complex64 if prec == 32 else torch.complex128
+        x = x.astype(dt)
+        tx = torch.from_numpy(x).to("cuda")
+        ty = torch.empty_like(tx)
+        t0 = time.perf_counter()
+        plan = Plan(shape, prec, None, direction)
+        plan_s = time.perf_counter() - t0
+
+        def run():
+            plan.execute_device(shape, tx.data_ptr(), ty.data_ptr(), stream.cuda_stream)
+
+        run()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        est = max(e0.elapsed_time(e1), 1e-3)
+        reps = int(max(5, min(200, 200.0 / est)))
+        e0.record(stream)
+        for _ in range(reps):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        y = ty.cpu().numpy()
+        xf = np.asfortranarray(x.reshape(shape, order="F"))
+        ref = oracle.fftn(xf.astype(np.complex128), None, direction) if len(shape) > 1 else \
+            oracle.fft(x.astype(np.complex128), direction)
+        err = oracle.relerr(y.reshape(shape, order="F") if len(shape) > 1 else y, ref)
+        bound = (1e-14 if prec == 64 else 2e-6) * math.log2(n)
+        esz = 8 if prec == 32 else 16
+        row = {
+            "config": name, "shape": list(shape), "dtype": "c32" if prec == 32 else "c64",
+            "gpu_us": round(ms * 1e3, 2), "gflops": round(5 * n * math.log2(n) / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(2 * n * esz / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "relerr_vs_fp64_oracle": float(f"{err:.3e}"), "bound": float(f"{bound:.3e}"),
+            "plan_ms": round(plan_s * 1e3, 1),
+        }
+        if cpu:
+            xs = x if len(shape) == 1 else xf
+            samples = []
+            while not samples or (sum(samples) < cpu_budget_s and len(samples) < 20):
+                t0 = time.perf_counter()
+                if len(shape) > 1:
+                    oracle.fftn(xs, None, direction)
+                else:
+                    oracle.fft(xs, direction)
+                samples.append(time.perf_counter() - t0)
+            cpu_s = float(np.median(samples))
+            row["cpu_us"] = round(cpu_s * 1e6, 1)
+            row["cpu_samples"] = len(samples)
+            row["gpu_vs_cpu"] = round(cpu_s / (ms * 1e-3), 1)
+        rows.append(row)
+        del tx, ty, plan
+    return rows
+
+
+def per_call_fp64(torch, lengths, dirs, sample: int = 2000, seed: int = 3):
+    """The plain per-call FP64 path on a random sample of the batched
+    workload: one mul! per transform (ComplexF64, plans built beforehand,
+    arrays resident), timed over the whole sample."""
+    from fftahip.plan import Plan
+
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(len(lengths), size=min(sample, len(lengths)), replace=False))
+    L = lengths[idx]
+    D = dirs[idx]
+    offs = np.concatenate([[0], np.cumsum(L)[:-1]]).astype(np.int64)
+    total = int(L.sum())
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    x = torch.randn(total, dtype=torch.complex128, device="cuda", generator=gen)
+    y = torch.empty_like(x)
+    plans = {}
+    for n, d in zip(L.tolist(), D.tolist()):
+        if (n, d) not in plans:
+            plans[(n, d)] = Plan((n,), 64, None, d)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    xp, yp = x.data_ptr(), y.data_ptr()
+    calls = [(plans[(n, d)], (n,), xp + 16 * o, yp + 16 * o) for n, d, o in zip(L.tolist(), D.tolist(), offs.tolist())]
+
+    def run():
+        for p, dims, a, b in calls:
+            p.execute_device(dims, a, b, sp)
+
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = e0.elapsed_time(e1)
+    return {"transforms": int(idx.size), "distinct_plans": len(plans),
+            "transforms_per_s": round(idx.size / (dev_ms * 1e-3), 1),
+            "wall_transforms_per_s": round(idx.size / wall, 1),
+            "sample": f"{idx.size} of {len(lengths)} transforms of the batched workload (seed {seed}), ComplexF64, "
+                      f"one fftahip_execute per transform on one stream"}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +360,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="length of the serving replay (0: skip)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the single-transform configurations")
+    ap.add_argument("--no-percall", action="store_true", help="skip the per-call FP64 comparison")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -300,6 +453,17 @@ def main(argv=None) -> int:
             "throughput_req_s": round(st["requests_per_s"], 1), "payload_GBps": round(st["payload_gbps"], 2),
             "pool_peak_bytes": st["pool_peak_bytes"],
         }
+    if not args.no_percall:
+        pc = per_call_fp64(torch, lengths, dirs)
+        batched_tps = args.count / (ms * 1e-3)
+        pc["batched_fp32_transforms_per_s"] = round(batched_tps, 1)
+        pc["batched_fp32_over_percall_fp64"] = round(batched_tps / pc["transforms_per_s"], 2)
+        line["percall_fp64"] = pc
+    if not args.no_configs:
+        line["per_call_configs"] = per_call_configs(torch, cpu=not args.no_cpu)
+        line["per_call_cpu"] = {"kind": "port", "cores": 1, "threads": "Julia threads: n/a (FFTA is single-threaded)",
+                                "what": "oracle (C++ restatement of FFTA's algorithm) in the plan's precision, "
+                                        "median of up to 20 calls within ~3 s"}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lengths, dirs)
     print(json.dumps(line), flush=True)

@@ -91,18 +91,25 @@ struct LinePass {
   const int64_t* in_tab;
   const int64_t* out_tab;
   const void* tw;  // per-stage twiddles of this radix schedule (kernels.hip, stage()), pass precision
-  // Pass twiddle: element t of a line is multiplied by
-  // w_N^{i_twdim * t * tw_scale}, w_N^m = lo[m & (2^sh-1)] * hi[m >> sh]
-  // (FP64 tables, product in FP64, rounded to the pass's precision),
-  // on load (pretw = 1, multi-pass Stockham) or on store (pretw = 2,
-  // four-step column/row passes).
+  // Pass twiddle: element t of line index k = i_twdim is multiplied by
+  // w_N^{k * t * tw_scale}, on load (pretw = 1, multi-pass Stockham) or on
+  // store (pretw = 2, four-step column/row passes).  The exponent is split
+  // so that neighbouring lanes read neighbouring table entries:
+  //   tw_split 0 (lanes run over lines):     k*t = (k & ~15)*t + (k & 15)*t
+  //   tw_split 1 (lanes run along the line): k*t = k*(t & ~15) + k*(t & 15)
+  // The first part is w_N^m = lo[m & (2^sh-1)] * hi[m >> sh] (FP64 tables,
+  // product in FP64), the same for 16 neighbouring lanes; the second is
+  // tw16[a*16 + j] = w_N^{a*j*tw_scale} (a = t or k, j < 16; FP64-generated,
+  // stored in the pass precision).  Their product is taken in the pass
+  // precision.
   int32_t pretw;
   int32_t twdim;
   int32_t tw_sh;
-  int32_t pad0_;
+  int32_t tw_split;
   int64_t tw_scale;
   const double2* tw_lo;
   const double2* tw_hi;
+  const void* tw16;
   int32_t conj_in, conj_out;
   double scale;  // multiply on store when != 1
   int32_t load_mode, mid_mode, store_mode;

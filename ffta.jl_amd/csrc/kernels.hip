@@ -259,13 +259,52 @@ __device__ __forceinline__ LineOffsets line_offsets(const LinePass& p, uint32_t 
   return o;
 }
 
-// Pass twiddle w_M^m from the two-level FP64 tables (product in FP64, then
-// rounded): m = twk * t * tw_scale < M < 2^31.
+// Pass twiddle w_M^{k*t*s} (s = tw_scale; k*t*s < M < 2^31) from its two
+// factors (see LinePass): requests, then the product.
+template <class Rt>
+struct PassTw {
+  double2 lo, hi;
+  cx<Rt> b;
+};
+
+template <class Rt>
+__device__ __forceinline__ PassTw<Rt> pass_twiddle_load(const LinePass& p, int64_t twk, int t) {
+  const uint32_t k = (uint32_t)twk, tt = (uint32_t)t;
+  uint32_t a, j, m;
+  if (p.tw_split == 0) {
+    j = k & 15u;
+    a = tt;
+    m = (k - j) * tt;
+  } else {
+    j = tt & 15u;
+    a = k;
+    m = k * (tt - j);
+  }
+  m *= (uint32_t)p.tw_scale;
+  const uint32_t mask = (1u << p.tw_sh) - 1u;
+  PassTw<Rt> w;
+  w.lo = gld(p.tw_lo + (m & mask));
+  w.hi = gld(p.tw_hi + (m >> p.tw_sh));
+  w.b = gld(reinterpret_cast<const cx<Rt>*>(p.tw16) + (a * 16u + j));
+  return w;
+}
+
+template <class Rt>
+__device__ __forceinline__ cx<Rt> pass_twiddle_value(const PassTw<Rt>& w) {
+  return cmul(from_d2<Rt>(dmul2(w.lo, w.hi)), w.b);
+}
+
 template <class Rt>
 __device__ __forceinline__ cx<Rt> pass_twiddle(const LinePass& p, int64_t twk, int t) {
-  const uint32_t m = (uint32_t)twk * (uint32_t)(t * p.tw_scale);
+  return pass_twiddle_value<Rt>(pass_twiddle_load<Rt>(p, twk, t));
+}
+
+// w_M^{k * s} in FP64 (two-level tables, FP64 product): the ratio of
+// consecutive pass twiddles along a line.
+__device__ __forceinline__ double2 step_twiddle(const LinePass& p, int64_t twk) {
+  const uint32_t m = (uint32_t)twk * (uint32_t)p.tw_scale;
   const uint32_t mask = (1u << p.tw_sh) - 1u;
-  return from_d2<Rt>(dmul2(gld(p.tw_lo + (m & mask)), gld(p.tw_hi + (m >> p.tw_sh))));
+  return dmul2(gld(p.tw_lo + (m & mask)), gld(p.tw_hi + (m >> p.tw_sh)));
 }
 
 template <class Rt, int EPT, int BLOCK, int MODE>
@@ -302,7 +341,6 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const cx<Rt>* __restrict__ chirp = reinterpret_cast<const cx<Rt>*>(p.chirp);
   const cx<Rt>* __restrict__ bhat = reinterpret_cast<const cx<Rt>*>(p.bhat);
   const int total = C * L;
-  const uint32_t twmask = (1u << p.tw_sh) - 1u;  // pass-twiddle exponents m < M < 2^31
   const uint32_t sgn_in = p.conj_in ? 0x80000000u : 0u;
   const uint32_t sgn_out = p.conj_out ? 0x80000000u : 0u;
   const bool chirp_in = MODE == MODE_FULL && p.load_mode == LOAD_CHIRP;
@@ -314,12 +352,72 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const bool fuse_out = MODE == MODE_PLAIN && p.fuse_out != 0;
   const bool has_mid = MODE == MODE_FULL && p.mid_mode == MID_BHAT;
 
+  // FP32 passes take the pass twiddles along a thread's run of positions as
+  // powers of one FP64 ratio (see reg_line); FP64 passes look each one up.
+  constexpr bool kGeo = sizeof(Rt) == 4;
+  // Column layout with C | BLOCK: every thread stays on one line (c = tid % C)
+  // and steps along it by BLOCK / C positions, so that offsets, LDS slots and
+  // pass twiddles advance incrementally instead of being re-derived.
+  const bool col_step = (BLOCK % C) == 0;
+  const int my_c = (int)(tid - fdiv((uint32_t)tid, p.divC) * (uint32_t)C);
+  const int my_t0 = (int)fdiv((uint32_t)tid, p.divC);
+  const int tstep = BLOCK / C;
+
   // ---- load (global -> LDS), coalesced across lines or along a line.  All
   // global loads of a batch (element, chirp factor, pass-twiddle halves) are
   // issued before any of them is consumed.
+  if (!fuse_in && p.col_in && col_step) {
+    if (my_c < nvalid) {
+      const int c = my_c;
+      const cx<Rt>* __restrict__ src = in + sh.in[c];
+      const int64_t cin0 = MODE == MODE_FULL ? sh.cin[c] : 0;
+      double2 w = make_double2(1.0, 0.0), wr = w;
+      // exponents k*t*s with t < L only (the tables end at M)
+      if (kGeo && pretw_in && my_t0 < L) {
+        w = step_twiddle(p, sh.twk[c] * (int64_t)my_t0);
+        if (tstep < L) wr = step_twiddle(p, sh.twk[c] * (int64_t)tstep);
+      }
+      for (int t0 = my_t0; t0 < L; t0 += tstep * U) {
+        cx<Rt> v[U], f[U];
+        PassTw<Rt> pw[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * tstep;
+          v[u] = {(Rt)0, (Rt)0};
+          f[u] = {(Rt)0, (Rt)0};
+          if (t < L) {
+            bool live = true;
+            if (chirp_in) {
+              const int64_t n = cin0 + (int64_t)t * p.cae;
+              live = n < p.chirp_n;
+              if (live) f[u] = gld(chirp + n);
+            }
+            if (live) v[u] = gld(src + (int64_t)t * p.ae);
+            if (!kGeo && pretw_in) pw[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * tstep;
+          if (t >= L) continue;
+          cx<Rt> x = cconj_if(v[u], sgn_in);
+          if (chirp_in) x = cmul(x, cconj(f[u]));
+          if (pretw_in) {
+            if (kGeo) {
+              x = cmul(x, from_d2<Rt>(w));
+              w = dmul2(w, wr);
+            } else {
+              x = cmul(x, pass_twiddle_value<Rt>(pw[u]));
+            }
+          }
+          lds[c * ldl + pad_idx(t, psh)] = x;
+        }
+      }
+    }
+  } else
   for (int e0 = 0; e0 < (fuse_in ? 0 : total); e0 += BLOCK * U) {
     cx<Rt> v[U], f[U];
-    double2 wl[U], wh[U];
+    PassTw<Rt> w[U];
     int slot[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -345,11 +443,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
             if (live) f[u] = gld(chirp + n);
           }
           if (live) v[u] = gld(in + sh.in[c] + (int64_t)t * p.ae);
-          if (pretw_in) {
-            const uint32_t m = (uint32_t)sh.twk[c] * (uint32_t)(t * p.tw_scale);
-            wl[u] = gld(p.tw_lo + (m & twmask));
-            wh[u] = gld(p.tw_hi + (m >> p.tw_sh));
-          }
+          if (pretw_in) w[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
         }
       }
     }
@@ -358,7 +452,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
       if (slot[u] < 0) continue;
       cx<Rt> x = cconj_if(v[u], sgn_in);
       if (chirp_in) x = cmul(x, cconj(f[u]));
-      if (pretw_in) x = cmul(x, from_d2<Rt>(dmul2(wl[u], wh[u])));
+      if (pretw_in) x = cmul(x, pass_twiddle_value<Rt>(w[u]));
       lds[slot[u]] = x;
     }
   }
@@ -383,17 +477,11 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     } else {
       x = cconj_if(gld(in + sh.in[cs] + (int64_t)e * p.ae), sgn_in);
     }
-    if (pretw_in) {
-      const uint32_t m = (uint32_t)sh.twk[cs] * (uint32_t)(e * p.tw_scale);
-      x = cmul(x, from_d2<Rt>(dmul2(gld(p.tw_lo + (m & twmask)), gld(p.tw_hi + (m >> p.tw_sh)))));
-    }
+    if (pretw_in) x = cmul(x, pass_twiddle<Rt>(p, sh.twk[cs], e));
     return x;
   };
   auto st = [&](int e, cx<Rt> x) {
-    if (pretw_out) {
-      const uint32_t m = (uint32_t)sh.twk[cs] * (uint32_t)(e * p.tw_scale);
-      x = cmul(x, from_d2<Rt>(dmul2(gld(p.tw_lo + (m & twmask)), gld(p.tw_hi + (m >> p.tw_sh)))));
-    }
+    if (pretw_out) x = cmul(x, pass_twiddle<Rt>(p, sh.twk[cs], e));
     if (store_mode != STORE_PLAIN) {
       const int64_t n = sh.cout[cs] + (int64_t)e * p.cbe;
       if (store_mode == STORE_BHAT) {
@@ -439,6 +527,18 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   // ---- store (LDS -> global).  As on the load side, the table factors of a
   // batch are all requested before the first store of the batch.
   if (MODE == MODE_PLAIN) {
+    if (!fuse_out && p.col_out && col_step) {
+      if (my_c < nvalid) {
+        cx<Rt>* __restrict__ dst = out + sh.out[my_c];
+        const cx<Rt>* line = lds + my_c * ldl;
+        for (int t = my_t0; t < L; t += tstep) {
+          cx<Rt> y = cconj_if(line[pad_idx(t, psh)], sgn_out);
+          if (has_scale) y = cscale(y, scale);
+          gst(dst + (int64_t)t * p.be, y);
+        }
+      }
+      return;
+    }
     for (int e0 = 0; e0 < (fuse_out ? 0 : total); e0 += BLOCK * U) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -460,9 +560,64 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     }
     return;
   }
+  if (p.col_out && col_step) {
+    if (my_c < nvalid) {
+      const int c = my_c;
+      cx<Rt>* __restrict__ dst = out + sh.out[c];
+      const int64_t cout0 = sh.cout[c];
+      double2 w = make_double2(1.0, 0.0), wr = w;
+      // exponents k*t*s with t < L only (the tables end at M)
+      if (kGeo && pretw_out && my_t0 < L) {
+        w = step_twiddle(p, sh.twk[c] * (int64_t)my_t0);
+        if (tstep < L) wr = step_twiddle(p, sh.twk[c] * (int64_t)tstep);
+      }
+      for (int t0 = my_t0; t0 < L; t0 += tstep * U) {
+        cx<Rt> x[U], f[U];
+        PassTw<Rt> pw[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * tstep;
+          ok[u] = t < L;
+          if (!ok[u]) continue;
+          if (store_mode != STORE_PLAIN) {
+            const int64_t n = cout0 + (int64_t)t * p.cbe;
+            if (store_mode == STORE_BHAT) {
+              f[u] = gld(bhat + n);
+            } else {
+              ok[u] = n < p.chirp_n;
+              f[u] = gld(chirp + (ok[u] ? n : 0));
+            }
+          }
+          if (!kGeo && pretw_out) pw[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
+          x[u] = lds[c * ldl + pad_idx(t, psh)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * tstep;
+          if (t >= L) continue;
+          cx<Rt> y = x[u];
+          if (pretw_out) {
+            if (kGeo) {
+              y = cmul(y, from_d2<Rt>(w));
+              w = dmul2(w, wr);
+            } else {
+              y = cmul(y, pass_twiddle_value<Rt>(pw[u]));
+            }
+          }
+          if (store_mode == STORE_BHAT) y = cconj(cmul(y, f[u]));
+          else if (store_mode == STORE_CHIRP) y = cconj(cmul(f[u], y));
+          y = cconj_if(y, sgn_out);
+          if (has_scale) y = cscale(y, scale);
+          if (ok[u]) gst(dst + (int64_t)t * p.be, y);
+        }
+      }
+    }
+    return;
+  }
   for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
     cx<Rt> x[U], f[U];
-    double2 wl[U], wh[U];
+    PassTw<Rt> w[U];
     int64_t o[U];
     bool ok[U];
 #pragma unroll
@@ -488,11 +643,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
           f[u] = gld(chirp + n);
         }
       }
-      if (pretw_out) {
-        const uint32_t m = (uint32_t)sh.twk[c] * (uint32_t)(t * p.tw_scale);
-        wl[u] = gld(p.tw_lo + (m & twmask));
-        wh[u] = gld(p.tw_hi + (m >> p.tw_sh));
-      }
+      if (pretw_out) w[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
       x[u] = lds[c * ldl + pad_idx(t, psh)];
       o[u] = sh.out[c] + (int64_t)t * p.be;
       ok[u] = true;
@@ -501,7 +652,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     for (int u = 0; u < U; ++u) {
       if (!ok[u]) continue;
       cx<Rt> y = x[u];
-      if (pretw_out) y = cmul(y, from_d2<Rt>(dmul2(wl[u], wh[u])));
+      if (pretw_out) y = cmul(y, pass_twiddle_value<Rt>(w[u]));
       if (store_mode == STORE_BHAT) y = cconj(cmul(y, f[u]));
       else if (store_mode == STORE_CHIRP) y = cconj(cmul(f[u], y));
       y = cconj_if(y, sgn_out);
@@ -581,20 +732,41 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
       v[t] = cconj_if(gld(in + (int64_t)t * p.ae), sgn_in);
     }
   }
+  // FP32: the pass twiddles of a line, w^{k*t*s} for t = 1..L-1, are the
+  // powers of w^{k*s}, formed by FP64 products (< 48 steps: error below
+  // 1e-14 relative, invisible after rounding to FP32); FP64 looks every
+  // factor up.
+  constexpr bool kGeo = sizeof(Rt) == 4;
+  double2 wr = make_double2(1.0, 0.0);
+  if (kGeo && (pretw_in || pretw_out)) wr = step_twiddle(p, o.twk);
   if (pretw_in) {
+    double2 w = wr;
 #pragma unroll
     for (int t = 1; t < L; ++t) {
-      v[t] = cmul(v[t], pass_twiddle<Rt>(p, o.twk, t));
-      if (t % kRegChunk == 0) __builtin_amdgcn_sched_barrier(0);
+      if (kGeo) {
+        v[t] = cmul(v[t], from_d2<Rt>(w));
+        w = dmul2(w, wr);
+      } else {
+        v[t] = cmul(v[t], pass_twiddle<Rt>(p, o.twk, t));
+        if (t % kRegChunk == 0) __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   dft_any<Rt, L>(v);
   const Rt scale = (Rt)p.scale;
   const bool has_scale = p.scale != 1.0;
+  double2 w = wr;
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     cx<Rt> y = v[k];
-    if (pretw_out && k > 0) y = cmul(y, pass_twiddle<Rt>(p, o.twk, k));
+    if (pretw_out && k > 0) {
+      if (kGeo) {
+        y = cmul(y, from_d2<Rt>(w));
+        w = dmul2(w, wr);
+      } else {
+        y = cmul(y, pass_twiddle<Rt>(p, o.twk, k));
+      }
+    }
     bool live = true;
     if (store_mode == STORE_BHAT) {
       y = cconj(cmul(y, gld(bhat + o.cout + (int64_t)k * p.cbe)));
@@ -612,12 +784,13 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
 
 // Line lengths of the register passes: every 13-smooth length up to
 // reg_max (one line's values, and the table factors in flight, must fit the
-// 128 registers): plain passes FP32 48 / FP64 24, load-side pass twiddle
-// FP32 32 / FP64 16, the full set of fused operations FP32 24 / FP64 12.
+// 128 registers): FP32 48 (24 with the full set of fused operations), FP64
+// 24 for plain passes, 16 with the load-side pass twiddle, 12 with the full
+// set.
 template <class Rt, int MODE>
 constexpr int reg_max() {
-  constexpr int f32 = MODE == MODE_PLAIN ? 48 : (MODE == MODE_PRETW ? 32 : 24);
-  return sizeof(Rt) == 4 ? f32 : f32 / 2;
+  if (sizeof(Rt) == 4) return MODE == MODE_FULL ? 24 : 48;
+  return MODE == MODE_PLAIN ? 24 : (MODE == MODE_PRETW ? 16 : 12);
 }
 
 template <class Rt, int MODE>
@@ -685,6 +858,16 @@ __global__ void gen_twiddle_2level_kernel(double2* lo, double2* hi, int64_t N, i
   const int64_t nlo = 1ll << sh;
   if (i < nlo) lo[i] = unit_root(i, N);
   if (i < nhi) hi[i] = unit_root(i << sh, N);
+}
+
+// tw16[a*16 + j] = w_N^{(a*j*s) mod N} for a < amax, j < 16 (second factor
+// of the pass twiddles), FP64 evaluated, stored in the pass precision.
+template <class Rt>
+__global__ void gen_tw16_kernel(cx<Rt>* out, int64_t N, int64_t s, int64_t amax) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= amax * 16) return;
+  const int64_t a = i >> 4, j = i & 15;
+  out[i] = from_d2<Rt>(unit_root(((a * j) % N) * s % N, N));
 }
 
 // Forward Bluestein chirp b_n = exp(+i pi p / N), p = n^2 mod 2N taken in
@@ -844,6 +1027,17 @@ hipError_t launch_gen_twiddle_2level(double2* lo, double2* hi, int64_t N, int sh
   const int64_t n = (1ll << sh) > nhi ? (1ll << sh) : nhi;
   hipLaunchKernelGGL(gen_twiddle_2level_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, lo, hi, N,
                      sh, nhi);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_tw16(void* out, int precision, int64_t N, int64_t s, int64_t amax, hipStream_t stream) {
+  const int64_t blocks = (amax * 16 + 255) / 256;
+  if (precision == 32)
+    hipLaunchKernelGGL(gen_tw16_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<cx<float>*>(out), N, s, amax);
+  else
+    hipLaunchKernelGGL(gen_tw16_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<cx<double>*>(out), N, s, amax);
   return hipGetLastError();
 }
 

@@ -85,7 +85,7 @@ static DevInfo& dev_info() {
 static thread_local bool g_dry = false;
 static thread_local int64_t g_dry_lds = 0;
 enum TokenKind { TK_STAGE = 1, TK_TWLO = 2, TK_TWHI = 3, TK_CHIRP = 4, TK_BHAT = 5, TK_IN = 6, TK_OUT = 7,
-                 TK_SCRATCH = 8, TK_BHATROWS = 9, TK_OFFTAB = 10 };
+                 TK_SCRATCH = 8, TK_BHATROWS = 9, TK_OFFTAB = 10, TK_TW16 = 11 };
 static void* token(int kind, int64_t v) {
   return (void*)(uintptr_t)((1ull << 60) | ((uint64_t)kind << 52) | (uint64_t)v);
 }
@@ -299,6 +299,7 @@ struct Tables {
   int prec = 32;
   std::map<std::vector<int64_t>, std::unique_ptr<DevBuf>> stage;  // (L, radices) -> per-stage twiddles
   std::map<int64_t, std::pair<std::unique_ptr<DevBuf>, int>> two;  // M -> (lo|hi, sh)
+  std::map<std::vector<int64_t>, std::unique_ptr<DevBuf>> tw16;     // (M, scale, amax) -> second factors
 };
 
 static size_t csize(int prec) { return prec == 32 ? 8 : 16; }
@@ -357,6 +358,26 @@ static fftahip_status two_level(Tables& T, int64_t M, const double2** lo, const 
   *lo = base;
   *hi = base + (1ll << sh);
   *sh_out = sh;
+  return FFTAHIP_SUCCESS;
+}
+
+// Second factors of the pass twiddles: tw16[a*16 + j] = w_M^{a*j*scale},
+// a < amax, j < 16 (cached per (M, scale, amax)).
+static fftahip_status tw16_table(Tables& T, int64_t M, int64_t scale, int64_t amax, const void** out, hipStream_t s) {
+  if (g_dry) {
+    *out = token(TK_TW16, amax * 16);
+    return FFTAHIP_SUCCESS;
+  }
+  const std::vector<int64_t> key = {M, scale, amax};
+  auto it = T.tw16.find(key);
+  if (it == T.tw16.end()) {
+    auto buf = std::make_unique<DevBuf>();
+    fftahip_status st = buf->ensure((size_t)(amax * 16) * csize(T.prec));
+    if (st) return st;
+    HIP_TRY(launch_gen_tw16(buf->p, T.prec, M, scale, amax, s));
+    it = T.tw16.emplace(key, std::move(buf)).first;
+  }
+  *out = it->second->p;
   return FFTAHIP_SUCCESS;
 }
 
@@ -758,6 +779,24 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   }
 }
 
+// Pass twiddle layout of a built pass: split the exponent along the index
+// that runs across neighbouring lanes (lines when the pass side is laid out
+// across lines or the pass runs in registers, else positions in the line),
+// and attach the matching second-factor table.
+static fftahip_status attach_pass_twiddle(LinePass& p, const Launch& la, Tables& T, int64_t M, hipStream_t s) {
+  if (!p.pretw) return FFTAHIP_SUCCESS;
+  const bool lanes_over_lines = la.reg || (p.pretw == 1 ? p.col_in != 0 : p.col_out != 0);
+  p.tw_split = lanes_over_lines ? 0 : 1;
+  int64_t ext;
+  if (p.twdim < 3) {
+    ext = p.n[p.twdim];
+  } else {
+    ext = p.nlines / std::max<int64_t>(1, p.n[0] * p.n[1] * p.n[2]);
+  }
+  const int64_t amax = p.tw_split == 0 ? (int64_t)p.L : ext;
+  return tw16_table(T, M, p.tw_scale, amax, &p.tw16, s);
+}
+
 static LinePass blank_pass() {
   LinePass p;
   std::memset(&p, 0, sizeof(p));
@@ -909,6 +948,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       st = stage_table(T, d->inner[k == 1 ? 1 : 0], &p.tw, stream);
       if (st) return st;
       set_inner(p, d->inner[k == 1 ? 1 : 0], prec, la);
+      if ((st = attach_pass_twiddle(p, la, T, M, stream))) return st;
       la.lp = p;
       L.push_back(la);
     }
@@ -995,6 +1035,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       for (int u = 0; u < 4; ++u) p.ca[u] = p.cb[u] = p.cm[u] = 0;
     }
     set_inner(p, d->inner[s], prec, la);
+    if ((st = attach_pass_twiddle(p, la, T, M, stream))) return st;
     la.lp = p;
     L.push_back(la);
   }
@@ -1921,7 +1962,9 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   json_ptr(os, "out_tab", p.out_tab);
   os << ", ";
   json_ptr(os, "tw", p.tw);
-  os << ", \"pretw\": " << p.pretw << ", \"twdim\": " << p.twdim << ", \"tw_sh\": " << p.tw_sh
+  os << ", ";
+  json_ptr(os, "tw16", p.tw16);
+  os << ", \"tw_split\": " << p.tw_split << ", \"pretw\": " << p.pretw << ", \"twdim\": " << p.twdim << ", \"tw_sh\": " << p.tw_sh
      << ", \"tw_scale\": " << p.tw_scale << ", ";
   json_ptr(os, "tw_lo", p.tw_lo);
   os << ", ";

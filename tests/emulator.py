@@ -14,7 +14,8 @@ from __future__ import annotations
 
 import numpy as np
 
-TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
+TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB, TK_TW16 = \
+    1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
 EPT = 16
 STATIC_LDS = 6 * 8 * 256
 # register line passes: lines per workgroup, and the lengths with a codelet
@@ -152,12 +153,23 @@ class Emulator:
                 continue
             self._pass(la, lds_per_block)
 
+    @staticmethod
+    def _check_tw16(p, k, L):
+        """The second twiddle factor is tw16[a*16 + j] with a = t (split 0) or
+        a = k (split 1): the table must cover every a the pass uses."""
+        tok = p["tw16"]
+        if tok is None or tok[0] != TK_TW16:
+            raise ScheduleError("pass twiddle without its second-factor table")
+        amax = L if p["tw_split"] == 0 else int(k.max()) + 1
+        if amax * 16 > tok[1]:
+            raise ScheduleError(f"pass twiddle second factor: needs {amax * 16} entries, table has {tok[1]}")
+
     def _pass(self, p, lds_per_block):
         L, C, Tl = p["L"], p["C"], p["Tl"]
         if p.get("reg"):
             # register line pass: one line per thread, blocks of REG_BLOCK lines
             mode = 2 if (p["load_mode"] or p["mid_mode"] or p["store_mode"] or p["pretw"] == 2) else int(p["pretw"] == 1)
-            lmax = {0: 48, 1: 32, 2: 24}[mode] // (1 if self.esize == 8 else 2)
+            lmax = ({0: 48, 1: 48, 2: 24} if self.esize == 8 else {0: 24, 1: 16, 2: 12})[mode]
             if L not in REG_LENGTHS or L > lmax:
                 raise ScheduleError(f"register pass of unsupported length {L}")
             if p["block"] != REG_BLOCK or p["lds"] != 0 or p["grid"] * REG_BLOCK < p["nlines"]:
@@ -247,6 +259,7 @@ class Emulator:
             m = idx[p["twdim"]][:, None] * t[None, :] * p["tw_scale"]
             if m.max() >= M:
                 raise ScheduleError("pass twiddle exponent out of range")
+            self._check_tw16(p, idx[p["twdim"]], L)
             v = v * np.exp(-2j * np.pi * m / M)
         v = np.fft.fft(v, axis=1)
         if p["mid_mode"] == 1:
@@ -260,6 +273,7 @@ class Emulator:
             m = idx[p["twdim"]][:, None] * t[None, :] * p["tw_scale"]
             if m.max() >= M:
                 raise ScheduleError("pass twiddle exponent out of range")
+            self._check_tw16(p, idx[p["twdim"]], L)
             v = v * np.exp(-2j * np.pi * m / M)
         addr_out = do + oout[:, None] + t[None, :] * p["be"]
         valid_out = np.ones_like(addr_out, dtype=bool)
