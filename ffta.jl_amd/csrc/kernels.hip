@@ -355,13 +355,14 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   // FP32 passes take the pass twiddles along a thread's run of positions as
   // powers of one FP64 ratio (see reg_line); FP64 passes look each one up.
   constexpr bool kGeo = sizeof(Rt) == 4;
-  // Column layout with C | BLOCK: every thread stays on one line (c = tid % C)
-  // and steps along it by BLOCK / C positions, so that offsets, LDS slots and
-  // pass twiddles advance incrementally instead of being re-derived.
-  const bool col_step = (BLOCK % C) == 0;
-  const int my_c = (int)(tid - fdiv((uint32_t)tid, p.divC) * (uint32_t)C);
-  const int my_t0 = (int)fdiv((uint32_t)tid, p.divC);
+  // Column layout: thread tid < C * (BLOCK / C) stays on one line
+  // (c = tid % C) and steps along it by BLOCK / C positions, so that offsets,
+  // LDS slots and pass twiddles advance incrementally instead of being
+  // re-derived; the remaining BLOCK % C threads sit out the load and store.
   const int tstep = BLOCK / C;
+  const bool col_step = tstep >= 1;
+  const int my_t0 = (int)fdiv((uint32_t)tid, p.divC);
+  const int my_c = my_t0 < tstep ? (int)(tid - (uint32_t)my_t0 * (uint32_t)C) : C;
 
   // ---- load (global -> LDS), coalesced across lines or along a line.  All
   // global loads of a batch (element, chirp factor, pass-twiddle halves) are

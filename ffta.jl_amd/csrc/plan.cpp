@@ -964,6 +964,12 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
   int tsh = 0;
   st = two_level(T, M, &tlo, &thi, &tsh, stream);
   if (st) return st;
+  // Lines adjacent in memory with strided positions (e.g. the second
+  // dimension of a matrix): the scratch keeps the line index fastest as well
+  // (element m of line b at b + B*m), so that every pass reads and writes
+  // runs of neighbouring lines.  Otherwise scratch holds line b at b*M.
+  const bool line_minor = !tab && strided && ls.n[0] > 1 && ls.in_str[0] == 1 && ls.out_str[0] == 1;
+  const int64_t es_scr = line_minor ? B : 1;
   for (int i = 0; i < total; ++i) {
     const int s = i % np;
     const int64_t R = d->F[s];
@@ -975,8 +981,8 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
     LinePass p = blank_pass();
     set_in(p, first ? in : (((i - 1) % 2 == 0) ? s0 : s1));
     set_out(p, last ? out : ((i % 2 == 0) ? s0 : s1));
-    const int64_t ein = first ? ls.es_in : 1;
-    const int64_t eout = last ? ls.es_out : 1;
+    const int64_t ein = first ? ls.es_in : es_scr;
+    const int64_t eout = last ? ls.es_out : es_scr;
     p.ae = ein * (M / R);
     p.be = eout * Ns;
     std::vector<Dim> D;
@@ -990,7 +996,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       p.in_tab = first ? ls.in_tab : nullptr;
       p.out_tab = last ? ls.out_tab : nullptr;
     } else {
-      int64_t sstride = M;
+      int64_t sstride = line_minor ? 1 : M;
       for (int b = 0; b < 2; ++b) {
         if (ls.n[b] > 1) {
           D.push_back(Dim{ls.n[b], first ? ls.in_str[b] : sstride, last ? ls.out_str[b] : sstride, 0, 0, false});
