@@ -351,6 +351,23 @@ def per_call_fp64(torch, lengths, dirs, sample: int = 2000, seed: int = 3):
                       f"one fftahip_execute per transform on one stream"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "line_pass_batched_uneven_c32_traffic.json")
+
+
+def measured_traffic(desc):
+    """HBM bytes per step from the committed rocprofv3 --pmc measurement
+    (FETCH_SIZE + WRITE_SIZE over one step's launches, scaled by the same
+    counters' reading for a 1 GiB copy), if it was taken on this schedule."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("launches_per_step") != len(desc.get("phases", [])):
+        return None
+    return t.get("bytes_per_step")
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +379,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="length of the serving replay (0: skip)")
     ap.add_argument("--no-configs", action="store_true", help="skip the single-transform configurations")
     ap.add_argument("--no-percall", action="store_true", help="skip the per-call FP64 comparison")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="after the steps, copy 1 GiB once (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -399,6 +418,14 @@ def main(argv=None) -> int:
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / args.steps
 
+    if args.probe_copy:
+        a = torch.zeros(1 << 27, dtype=torch.complex64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     flops = flops_of(lengths)
     bytes_step = 2.0 * total * 8
     achieved = bytes_step / (ms * 1e-3) / 1e9
@@ -433,7 +460,7 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(desc),
         },
         "error": {"worst_relerr_over_bound": round(worst, 4), "sample": nsample,
                   "bound": "2e-6*log2(N) vs FP64 oracle"},
