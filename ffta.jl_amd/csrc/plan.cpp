@@ -1251,6 +1251,7 @@ struct fftahip_plan_s {
   int prec = 32;
   int dir = -1;
   int64_t cutoff = 73;
+  double scale = 1.0;                             // applied by the pass that writes the result
   std::vector<int64_t> region;                    // sorted, 1-based
   std::vector<std::unique_ptr<LengthPlan>> lens;  // one per region dim
   Tables tables;
@@ -1379,6 +1380,16 @@ fftahip_status fftahip_plan_precision(fftahip_plan plan, int* out) {
 fftahip_status fftahip_plan_direction(fftahip_plan plan, int* out) {
   if (!plan || !out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
   *out = plan->dir;
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_plan_set_scale(fftahip_plan plan, double scale) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  if (!std::isfinite(scale)) return fail(FFTAHIP_ERR_ARGUMENT, "scale must be finite");
+  if (scale != plan->scale) {
+    plan->scale = scale;
+    plan->scheds.clear();  // schedules carry the factor in their last pass
+  }
   return FFTAHIP_SUCCESS;
 }
 
@@ -1518,6 +1529,16 @@ static fftahip_status plan_schedule(fftahip_plan plan, int ndims, const int64_t*
     st = build_launches(*plan->lens[k], plan->tables, ls, src, BufRef{BUF_OUT, 0}, BufRef{BUF_SCRATCH, 0},
                         BufRef{BUF_SCRATCH, per * ls.count()}, plan->dir == 1, L, s);
     if (st) return st;
+  }
+  // a scaled plan (ifft's 1/N, AbstractFFTs' ScaledPlan) scales in the last
+  // pass: it writes every element of the result once, after all earlier ones
+  if (plan->scale != 1.0) {
+    Launch* last = nullptr;
+    for (auto& la : L)
+      if (la.op == OP_PASS) last = &la;
+    if (!last)
+      return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "a scale factor on a transform without a pass (all region lengths 1)");
+    last->lp.scale *= plan->scale;
   }
   *scratch_need = need;
   return FFTAHIP_SUCCESS;

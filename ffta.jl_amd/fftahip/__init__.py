@@ -11,8 +11,9 @@ Reference correspondence:
   Plan.size(i)              src/plan.jl:36-47
   mul_(y, p, x)             LinearAlgebra.mul!, src/plan.jl:157-285
   p * x                     Base.:*, src/plan.jl:374-384
-  fft / bfft / ifft         AbstractFFTs convenience front-ends (plan, then apply;
-                            ifft = bfft scaled by 1/prod(size) over the region)
+  plan_ifft / ifft          AbstractFFTs ScaledPlan: bfft with 1/prod(size(x, region))
+                            applied by the pass that writes the result
+  fft / bfft                AbstractFFTs convenience front-ends (plan, then apply)
 Errors map to DimensionMismatch, ArgumentError and DomainError with the
 reference's messages.
 """
@@ -29,6 +30,7 @@ from .plan import (  # noqa: F401
     Batch,
     plan_fft,
     plan_bfft,
+    plan_ifft,
     fft,
     bfft,
     ifft,

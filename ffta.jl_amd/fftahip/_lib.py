@@ -37,6 +37,7 @@ SIGNATURES = {
     "fftahip_plan_rank": (_st, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "fftahip_plan_precision": (_st, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "fftahip_plan_direction": (_st, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "fftahip_plan_set_scale": (_st, [_vp, ctypes.c_double]),
     "fftahip_plan_check": (_st, [_vp, ctypes.c_int, _i64p, ctypes.c_int, _i64p, ctypes.c_int, _i64p]),
     "fftahip_execute": (_st, [_vp, ctypes.c_int, _i64p, _vp, _vp, _vp]),
     "fftahip_execute_host": (_st, [_vp, ctypes.c_int, _i64p, _vp, _vp]),

@@ -111,6 +111,8 @@ class Plan:
         L = load()
         _check(L.fftahip_plan_create(ctypes.byref(self._h), precision, direction, len(dims), i64_array(dims),
                                      len(reg), i64_array(reg), cutoff))
+        if scale != 1.0:
+            _check(L.fftahip_plan_set_scale(self._h, float(scale)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -230,16 +232,23 @@ def bfft(x, region=None, **kw) -> np.ndarray:
     return plan_bfft(a, region, **kw) * a
 
 
-def ifft(x, region=None, **kw) -> np.ndarray:
-    """AbstractFFTs ifft: bfft scaled by 1/prod(size(x, region))."""
-    a = _complexify(x)
-    reg = _region_tuple(region, a.ndim)
+def plan_ifft(x, region=None, BLUESTEIN_CUTOFF: int = DEFAULT_BLUESTEIN_CUTOFF, **_kw) -> Plan:
+    """AbstractFFTs.plan_ifft: plan_bfft scaled by 1/prod(size(x, region)), the
+    factor applied by the pass that writes the result."""
+    a = _complexify(x) if not isinstance(x, DeviceArray) else x
+    prec = a.precision if isinstance(a, DeviceArray) else _precision_of(a.dtype)
+    shape = a.shape
+    reg = _region_tuple(region, len(shape))
     n = 1
     for r in reg:
-        n *= a.shape[r - 1] if r <= a.ndim else 1
-    y = bfft(a, region, **kw)
-    y *= (1.0 / n)
-    return y
+        n *= shape[r - 1] if r <= len(shape) else 1
+    return Plan(shape, prec, region, FFT_BACKWARD, BLUESTEIN_CUTOFF, scale=1.0 / n)
+
+
+def ifft(x, region=None, **kw) -> np.ndarray:
+    """AbstractFFTs ifft = plan_ifft(x, region) * x."""
+    a = _complexify(x)
+    return plan_ifft(a, region, **kw) * a
 
 
 class Batch:

@@ -72,6 +72,16 @@ AbstractFFTs.plan_fft(x::AbstractArray{T}, region; kwargs...) where {T<:Union{Co
 AbstractFFTs.plan_bfft(x::AbstractArray{T}, region; kwargs...) where {T<:Union{ComplexF32,ComplexF64}} =
     _plan(x, region, FFT_BACKWARD; kwargs...)
 
+# AbstractFFTs.plan_ifft = plan_bfft scaled by 1/prod(size(x, region)); the
+# factor is applied by the pass that writes the result (one sweep less than
+# scaling afterwards)
+function AbstractFFTs.plan_ifft(x::AbstractArray{T}, region; kwargs...) where {T<:Union{ComplexF32,ComplexF64}}
+    p = _plan(x, region, FFT_BACKWARD; kwargs...)
+    check(ccall((:fftahip_plan_set_scale, libfftahip), Cint, (Ptr{Cvoid}, Cdouble), p.handle,
+                inv(prod(size(x, d) for d in region))))
+    return p
+end
+
 # src/plan.jl:36-47
 function Base.size(p::FFTAHipPlan{T,N}, i::Int) where {T,N}
     out = Ref{Int64}(0)

@@ -78,6 +78,12 @@ fftahip_status fftahip_plan_rank(fftahip_plan plan, int* rank_out);
 fftahip_status fftahip_plan_precision(fftahip_plan plan, int* precision_out);
 fftahip_status fftahip_plan_direction(fftahip_plan plan, int* direction_out);
 
+/* Multiply every result of the plan by `scale`, fused into the pass that
+ * writes the output (AbstractFFTs' ScaledPlan; ifft = bfft scaled by
+ * 1/prod(size(x, region)), the normalisation AbstractFFTs applies on top of
+ * FFTA's plan_bfft, src/plan.jl:74-75).  Default 1. */
+fftahip_status fftahip_plan_set_scale(fftahip_plan plan, double scale);
+
 /* The argument checks of LinearAlgebra.mul!(y, p, x) (src/plan.jl:157-164,
  * 170-182, 210-222, 260-272), with the plan's region as the caller holds it
  * (`region`, nregion entries; FFTA plans carry a mutable region).  Returns

@@ -110,3 +110,54 @@ def test_ndim_333_table(torch_cuda):
     print("case ndim_333", flush=True)
     out = _gpu_transform(torch_cuda, y, -1)
     assert np.max(np.abs(out - ref)) <= 1e-14
+
+
+# ---------------------------------------------------------------- scaled plans (ifft)
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("shape,region", [((4096,), None), ((1009,), None), ((20011,), None), ((3199,), None),
+                                          ((12297,), None), ((64, 80), None), ((5, 6, 7), (1, 3)),
+                                          ((3000, 9), 2), ((1,), None)])
+def test_plan_ifft_fused_scale(torch_cuda, shape, region, precision):
+    """plan_ifft = plan_bfft with 1/prod(size(x, region)) applied by the pass
+    that writes the result (AbstractFFTs ScaledPlan semantics): equals the
+    oracle's bfft scaled by 1/N, and ifft(fft(x)) returns x."""
+    from fftahip.plan import Plan, mul_, plan_ifft
+
+    dt = np.complex64 if precision == 32 else np.complex128
+    x = _rand(shape, dt, seed=sum(shape) + precision)
+    print(f"case ifft shape={shape} region={region} prec={precision}", flush=True)
+    p = plan_ifft(x, region)
+    tx, dx = to_device(torch_cuda, x)
+    ty, dy = empty_device(torch_cuda, x.shape, x.dtype)
+    mul_(dy, p, dx)
+    y = to_host(torch_cuda, ty, x.shape)
+    reg = region if region is not None else tuple(range(1, len(shape) + 1))
+    reg = (reg,) if isinstance(reg, int) else reg
+    n = int(np.prod([shape[r - 1] for r in reg]))
+    ref = _oracle(x, 1, region) / n
+    assert oracle.relerr(y, ref) <= tol(n, precision)
+    # round trip through the forward plan
+    f = Plan(x.shape, precision, region, -1)
+    tz, dz = empty_device(torch_cuda, x.shape, x.dtype)
+    mul_(dz, f, dx)
+    mul_(dy, p, dz)
+    back = to_host(torch_cuda, ty, x.shape)
+    assert oracle.relerr(back, x.astype(np.complex128)) <= 2 * tol(n, precision)
+
+
+def test_set_scale_changes_result(torch_cuda):
+    """A scale factor set after a shape has run rebuilds that shape's schedule."""
+    from fftahip.plan import Plan, mul_
+    from fftahip import _lib
+
+    x = _rand((4099,), np.complex64, seed=3)
+    p = Plan(x.shape, 32, None, -1)
+    tx, dx = to_device(torch_cuda, x)
+    ty, dy = empty_device(torch_cuda, x.shape, x.dtype)
+    mul_(dy, p, dx)
+    y1 = to_host(torch_cuda, ty, x.shape)
+    _lib.load().fftahip_plan_set_scale(p.handle, 0.25)
+    mul_(dy, p, dx)
+    y2 = to_host(torch_cuda, ty, x.shape)
+    assert oracle.relerr(y2, 0.25 * y1.astype(np.complex128)) <= 1e-7
