@@ -136,4 +136,16 @@ struct BufPtrs {
   void* scratch;
 };
 
+// Element space of the real-transform boundary kernels: lines along one
+// array dimension, indexed (p, t, q) with p fastest.  p runs over the
+// dimensions before the transformed one (stride 1 on both sides), t along
+// the line, q over the dimensions after it.  Offsets are in elements of
+// each side's own type (real or complex).
+struct RealGeom {
+  int64_t P, T, Q;          // extents
+  int64_t in_t, in_q;       // input strides of t and q
+  int64_t out_t, out_q;     // output strides of t and q
+  FastDiv divP, divT;
+};
+
 }  // namespace fftahip

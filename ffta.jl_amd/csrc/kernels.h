@@ -21,6 +21,12 @@ hipError_t launch_reg_pass_groups(const LinePass* d_passes, const uint32_t* d_ti
 hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t L, hipStream_t stream);
 hipError_t launch_gen_twiddle_2level(double2* lo, double2* hi, int64_t N, int sh, int64_t nhi, hipStream_t stream);
 hipError_t launch_gen_tw16(void* out, int precision, int64_t N, int64_t s, int64_t amax, hipStream_t stream);
+// Real-transform boundary kernels (see kernels.hip); lo/hi/sh are the
+// two-level FP64 twiddle tables of n for the split steps.
+enum RealOp { REAL_PACK = 0, REAL_UNPACK, REAL_COMPLEXIFY, REAL_PART, REAL_COPY, REAL_HERMITIAN, REAL_RFFT_SPLIT,
+              REAL_BRFFT_SPLIT };
+hipError_t launch_real_boundary(int op, int precision, void* dst, const void* src, const RealGeom& g, int64_t m,
+                                const double2* lo, const double2* hi, int sh, int64_t n, hipStream_t s);
 hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_t N, hipStream_t stream);
 hipError_t launch_gen_chirp_ext(double2* bext, const double2* chirp64, int64_t N, int64_t M, hipStream_t stream);
 hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, hipStream_t stream);

@@ -871,6 +871,156 @@ __global__ void gen_tw16_kernel(cx<Rt>* out, int64_t N, int64_t s, int64_t amax)
   out[i] = from_d2<Rt>(unit_root(((a * j) % N) * s % N, N));
 }
 
+// ------------------------------------------------------------------ real transforms
+// Boundary kernels of rfft / brfft (src/plan.jl:386-500): even lengths
+// n = 2m run one complex FFT of m points on the packed pairs (x[2t],
+// x[2t+1]) with a split step before (brfft) or after (rfft) it (Sorensen et
+// al., eqs. 6-9, as FFTA cites); odd lengths run a complex FFT of n points
+// on the complexified input / Hermitian extension.  One thread per element
+// of the (p, t, q) space, p fastest, so neighbouring threads touch
+// neighbouring elements on both sides.
+
+__device__ __forceinline__ void real_geom_index(const RealGeom& g, uint64_t i, int64_t& p, int64_t& t, int64_t& q) {
+  const uint32_t a = fdiv((uint32_t)i, g.divP);
+  p = (int64_t)((uint32_t)i - a * (uint32_t)g.P);
+  const uint32_t b = fdiv(a, g.divT);
+  t = (int64_t)(a - b * (uint32_t)g.T);
+  q = (int64_t)b;
+}
+
+// w_n^{m'} for 0 <= m' < n from the two-level FP64 tables of n.
+__device__ __forceinline__ double2 twiddle_2l(const double2* lo, const double2* hi, int sh, int64_t m) {
+  const int64_t mask = (1ll << sh) - 1;
+  return dmul2(gld(lo + (m & mask)), gld(hi + (m >> sh)));
+}
+
+// A[p,t,q] = x[p, 2t, q] + i x[p, 2t+1, q]   (T = m pairs)
+template <class Rt>
+__global__ void real_pack_kernel(cx<Rt>* A, const Rt* x, RealGeom g) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, t, q;
+  real_geom_index(g, i, p, t, q);
+  const Rt* src = x + p + q * g.in_q + 2 * t * g.in_t;
+  A[p + t * g.out_t + q * g.out_q] = {src[0], src[g.in_t]};
+}
+
+// y[p, 2t, q] = Re B[p,t,q], y[p, 2t+1, q] = Im B[p,t,q]
+template <class Rt>
+__global__ void real_unpack_kernel(Rt* y, const cx<Rt>* B, RealGeom g) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, t, q;
+  real_geom_index(g, i, p, t, q);
+  const cx<Rt> v = B[p + t * g.in_t + q * g.in_q];
+  Rt* dst = y + p + q * g.out_q + 2 * t * g.out_t;
+  dst[0] = v.x;
+  dst[g.out_t] = v.y;
+}
+
+// A[p,t,q] = x[p,t,q] + 0i   (odd rfft)
+template <class Rt>
+__global__ void real_complexify_kernel(cx<Rt>* A, const Rt* x, RealGeom g) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, t, q;
+  real_geom_index(g, i, p, t, q);
+  A[p + t * g.out_t + q * g.out_q] = {x[p + t * g.in_t + q * g.in_q], (Rt)0};
+}
+
+// y[p,t,q] = Re B[p,t,q]   (odd brfft)
+template <class Rt>
+__global__ void real_part_kernel(Rt* y, const cx<Rt>* B, RealGeom g) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, t, q;
+  real_geom_index(g, i, p, t, q);
+  y[p + t * g.out_t + q * g.out_q] = B[p + t * g.in_t + q * g.in_q].x;
+}
+
+// Y[p,k,q] = B[p,k,q] for k <= n/2   (odd rfft: T = n/2 + 1)
+template <class Rt>
+__global__ void copy_lines_kernel(cx<Rt>* Y, const cx<Rt>* B, RealGeom g) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, t, q;
+  real_geom_index(g, i, p, t, q);
+  Y[p + t * g.out_t + q * g.out_q] = B[p + t * g.in_t + q * g.in_q];
+}
+
+// A[p,t,q] = X[p,t,q] for t <= n/2, conj(X[p,n-t,q]) above   (odd brfft: T = n)
+template <class Rt>
+__global__ void hermitian_extend_kernel(cx<Rt>* A, const cx<Rt>* X, RealGeom g, int64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, t, q;
+  real_geom_index(g, i, p, t, q);
+  const int64_t s = t <= n / 2 ? t : n - t;
+  const cx<Rt> v = X[p + s * g.in_t + q * g.in_q];
+  A[p + t * g.out_t + q * g.out_q] = t <= n / 2 ? v : cconj(v);
+}
+
+// rfft split step, in place on Y (m + 1 entries along t; Y[0..m-1] hold the
+// m-point FFT Z of the packed pairs): for k = 0..m/2 (T = m/2 + 1)
+//   Y[0] = Re Z0 + Im Z0, Y[m] = Re Z0 - Im Z0;
+//   XX = (Z_k + conj Z_{m-k}) / 2, XY = i (conj Z_{m-k} - Z_k) / 2,
+//   Y[k] = XX + w^k XY, Y[m-k] = conj(XX - w^k XY),  w = exp(-2 pi i / n).
+template <class Rt>
+__global__ void rfft_split_kernel(cx<Rt>* Y, RealGeom g, int64_t m, const double2* lo, const double2* hi, int sh) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, k, q;
+  real_geom_index(g, i, p, k, q);
+  cx<Rt>* line = Y + p + q * g.out_q;
+  if (k == 0) {
+    const cx<Rt> z = line[0];
+    line[0] = {z.x + z.y, (Rt)0};
+    line[m * g.out_t] = {z.x - z.y, (Rt)0};
+    return;
+  }
+  const cx<Rt> a = line[k * g.out_t];
+  const cx<Rt> b = cconj(line[(m - k) * g.out_t]);
+  const Rt h = (Rt)0.5;
+  const cx<Rt> xx = {h * (a.x + b.x), h * (a.y + b.y)};
+  const cx<Rt> d = {h * (b.x - a.x), h * (b.y - a.y)};
+  const cx<Rt> xy = {-d.y, d.x};  // i * d
+  const cx<Rt> w = from_d2<Rt>(twiddle_2l(lo, hi, sh, k));
+  const cx<Rt> wxy = cmul(w, xy);
+  // k = m/2 pairs with itself: FFTA's second assignment wins (src/plan.jl:438-439)
+  if (2 * k != m) line[k * g.out_t] = cadd(xx, wxy);
+  line[(m - k) * g.out_t] = cconj(csub(xx, wxy));
+}
+
+// brfft split step (X has m + 1 entries along t; A gets m), for j = 0..m/2:
+//   A[0] = (Re X0 + Re Xm) + i (Re X0 - Re Xm);
+//   XX = X_j + conj X_{m-j}, XY = w^j (X_j - conj X_{m-j}),  w = exp(+2 pi i / n),
+//   A[j] = XX + i XY, A[m-j] = conj(XX - i XY).
+template <class Rt>
+__global__ void brfft_split_kernel(cx<Rt>* A, const cx<Rt>* X, RealGeom g, int64_t m, const double2* lo,
+                                   const double2* hi, int sh, int64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)(g.P * g.T * g.Q)) return;
+  int64_t p, j, q;
+  real_geom_index(g, i, p, j, q);
+  const cx<Rt>* src = X + p + q * g.in_q;
+  cx<Rt>* dst = A + p + q * g.out_q;
+  if (j == 0) {
+    const Rt x0 = src[0].x, xm = src[m * g.in_t].x;
+    dst[0] = {x0 + xm, x0 - xm};
+    return;
+  }
+  const cx<Rt> a = src[j * g.in_t];
+  const cx<Rt> b = cconj(src[(m - j) * g.in_t]);
+  const cx<Rt> xx = cadd(a, b);
+  // w^j with w = exp(+2 pi i / n) is exp(-2 pi i (n - j) / n)
+  const cx<Rt> w = from_d2<Rt>(twiddle_2l(lo, hi, sh, n - j));
+  const cx<Rt> xy = cmul(w, csub(a, b));
+  const cx<Rt> ixy = {-xy.y, xy.x};
+  // j = m/2 pairs with itself: the second assignment wins (src/plan.jl:481-482)
+  if (2 * j != m) dst[j * g.out_t] = cadd(xx, ixy);
+  dst[(m - j) * g.out_t] = cconj(csub(xx, ixy));
+}
+
 // Forward Bluestein chirp b_n = exp(+i pi p / N), p = n^2 mod 2N taken in
 // (-N, N] (src/algos.jl:353-359 with sgn = +1), FP64 evaluated.
 template <class Rt>
@@ -1039,6 +1189,50 @@ hipError_t launch_gen_tw16(void* out, int precision, int64_t N, int64_t s, int64
   else
     hipLaunchKernelGGL(gen_tw16_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
                        reinterpret_cast<cx<double>*>(out), N, s, amax);
+  return hipGetLastError();
+}
+
+hipError_t launch_real_boundary(int op, int precision, void* dst, const void* src, const RealGeom& g, int64_t m,
+                                const double2* lo, const double2* hi, int sh, int64_t n, hipStream_t s) {
+  const uint64_t total = (uint64_t)(g.P * g.T * g.Q);
+  if (total == 0) return hipSuccess;
+  const dim3 grid((uint32_t)((total + 255) / 256)), block(256);
+#define FFTAHIP_REAL(Rt)                                                                                           \
+  switch (op) {                                                                                                   \
+    case REAL_PACK:                                                                                               \
+      hipLaunchKernelGGL(real_pack_kernel<Rt>, grid, block, 0, s, (cx<Rt>*)dst, (const Rt*)src, g);               \
+      break;                                                                                                      \
+    case REAL_UNPACK:                                                                                             \
+      hipLaunchKernelGGL(real_unpack_kernel<Rt>, grid, block, 0, s, (Rt*)dst, (const cx<Rt>*)src, g);             \
+      break;                                                                                                      \
+    case REAL_COMPLEXIFY:                                                                                         \
+      hipLaunchKernelGGL(real_complexify_kernel<Rt>, grid, block, 0, s, (cx<Rt>*)dst, (const Rt*)src, g);         \
+      break;                                                                                                      \
+    case REAL_PART:                                                                                               \
+      hipLaunchKernelGGL(real_part_kernel<Rt>, grid, block, 0, s, (Rt*)dst, (const cx<Rt>*)src, g);               \
+      break;                                                                                                      \
+    case REAL_COPY:                                                                                               \
+      hipLaunchKernelGGL(copy_lines_kernel<Rt>, grid, block, 0, s, (cx<Rt>*)dst, (const cx<Rt>*)src, g);          \
+      break;                                                                                                      \
+    case REAL_HERMITIAN:                                                                                          \
+      hipLaunchKernelGGL(hermitian_extend_kernel<Rt>, grid, block, 0, s, (cx<Rt>*)dst, (const cx<Rt>*)src, g, n); \
+      break;                                                                                                      \
+    case REAL_RFFT_SPLIT:                                                                                         \
+      hipLaunchKernelGGL(rfft_split_kernel<Rt>, grid, block, 0, s, (cx<Rt>*)dst, g, m, lo, hi, sh);               \
+      break;                                                                                                      \
+    case REAL_BRFFT_SPLIT:                                                                                        \
+      hipLaunchKernelGGL(brfft_split_kernel<Rt>, grid, block, 0, s, (cx<Rt>*)dst, (const cx<Rt>*)src, g, m, lo,  \
+                         hi, sh, n);                                                                              \
+      break;                                                                                                      \
+    default:                                                                                                      \
+      return hipErrorInvalidValue;                                                                                \
+  }
+  if (precision == 32) {
+    FFTAHIP_REAL(float)
+  } else {
+    FFTAHIP_REAL(double)
+  }
+#undef FFTAHIP_REAL
   return hipGetLastError();
 }
 

@@ -1598,6 +1598,303 @@ fftahip_status fftahip_execute_host(fftahip_plan plan, int ndims, const int64_t*
   return FFTAHIP_SUCCESS;
 }
 
+// ------------------------------------------------------------------ real transforms
+// plan_rfft / plan_brfft (src/plan.jl:102-150) and their application
+// (src/plan.jl:386-583).  1 region dim d1: even n = 2m runs an m-point
+// complex FFT on packed pairs with the split step (rfft after it, brfft
+// before it); odd n runs an n-point complex FFT on the complexified input
+// (rfft, keeping n/2 + 1 outputs) or on the Hermitian extension (brfft,
+// keeping the real part).  2 region dims d1 < d2: rfft along d1, then a
+// complex FFT along d2 in place; brfft: a complex backward FFT along d2 into
+// scratch, then brfft along d1.
+
+struct fftahip_rplan_s {
+  int prec = 32;
+  int dir = -1;  // -1 rfft (real in), +1 brfft (complex in, real out)
+  int64_t n = 1; // real length along d1
+  int d1 = 0, d2 = -1;
+  std::vector<int64_t> in_dims, out_dims;
+  std::unique_ptr<LengthPlan> lp1, lp2;
+  Tables tables;
+  DevSchedule S1, S2;
+  DevBuf scratch;
+  DevBuf hin, hout;
+  // scratch regions (complex elements)
+  int64_t offA = 0, offB = 0, offC = 0;
+  RealGeom pre{}, post{};
+  int pre_op = -1, post_op = -1;
+  int64_t m = 0;  // complex line length of S1
+  const double2* lo = nullptr;
+  const double2* hi = nullptr;
+  int sh = 0;
+  bool s1_from_c = false;  // 2-D brfft: S1's split step reads the d2-transformed copy
+};
+
+static RealGeom real_geom(int64_t P, int64_t T, int64_t Q, int64_t in_t, int64_t in_q, int64_t out_t,
+                          int64_t out_q) {
+  RealGeom g;
+  g.P = P;
+  g.T = T;
+  g.Q = Q;
+  g.in_t = in_t;
+  g.in_q = in_q;
+  g.out_t = out_t;
+  g.out_q = out_q;
+  g.divP = make_fastdiv((uint32_t)std::max<int64_t>(P, 1));
+  g.divT = make_fastdiv((uint32_t)std::max<int64_t>(T, 1));
+  return g;
+}
+
+static LineSet line_set(int64_t P, int64_t Q, int64_t in_es, int64_t in_q, int64_t out_es, int64_t out_q) {
+  LineSet ls;
+  ls.n[0] = P;
+  ls.in_str[0] = ls.out_str[0] = 1;
+  ls.n[1] = Q;
+  ls.in_str[1] = in_q;
+  ls.out_str[1] = out_q;
+  ls.es_in = in_es;
+  ls.es_out = out_es;
+  return ls;
+}
+
+fftahip_status fftahip_rplan_create(fftahip_rplan* plan, int precision, int direction, int ndims,
+                                    const int64_t* dims, int64_t len, int nregion, const int64_t* region,
+                                    int64_t bluestein_cutoff) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "plan pointer is NULL");
+  *plan = nullptr;
+  if (precision != 32 && precision != 64) return fail(FFTAHIP_ERR_ARGUMENT, "precision must be 32 or 64");
+  if (direction != -1 && direction != 1) return fail(FFTAHIP_ERR_ARGUMENT, "direction must be -1 or +1");
+  if (nregion != 1 && nregion != 2) return fail(FFTAHIP_ERR_ARGUMENT, "only supports 1D and 2D FFTs");
+  std::vector<int64_t> reg(region, region + nregion);
+  std::sort(reg.begin(), reg.end());
+  for (int64_t r : reg)
+    if (r < 1) return fail(FFTAHIP_ERR_ARGUMENT, "arraysize: dimension out of range");
+  // dims padded with 1s up to the largest region dim (size(x, r) == 1 beyond ndims)
+  int nd = ndims;
+  for (int64_t r : reg) nd = std::max<int>(nd, (int)r);
+  std::vector<int64_t> D(nd, 1);
+  for (int i = 0; i < ndims; ++i) D[i] = dims[i];
+  for (int64_t v : D)
+    if (v < 0) return fail(FFTAHIP_ERR_ARGUMENT, "negative dimension");
+  auto p = std::make_unique<fftahip_rplan_s>();
+  p->prec = precision;
+  p->dir = direction;
+  p->tables.prec = precision;
+  p->d1 = (int)reg[0] - 1;
+  p->d2 = nregion == 2 ? (int)reg[1] - 1 : -1;
+  const int d1 = p->d1, d2 = p->d2;
+  const int64_t n = direction == -1 ? D[d1] : len;
+  if (n <= 0) return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Array length must be strictly positive");
+  p->n = n;
+  const int64_t h = n / 2 + 1;
+  std::vector<int64_t> X = D, Y = D;  // input / output shapes
+  if (direction == -1) {
+    Y[d1] = h;
+  } else {
+    if (D[d1] != h) {
+      std::ostringstream os;
+      if (nregion == 1)
+        os << "real 1D plan has size " << n << ". Dimension of input array along region " << d1 + 1
+           << " should have size " << h << ", but has size " << D[d1];
+      else
+        os << "real 2D plan has size (" << n << ", " << D[d2] << "). Transform dimensions of input array are ("
+           << D[d1] << ", " << D[d2] << ") but should be (" << h << ", " << D[d2] << ")";
+      return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, os.str());
+    }
+    Y[d1] = n;
+  }
+  if (d2 >= 0 && D[d2] <= 0) return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Array length must be strictly positive");
+  p->in_dims = X;
+  p->out_dims = Y;
+  int64_t P = 1, Q = 1;
+  for (int i = 0; i < d1; ++i) P *= D[i];
+  for (int i = d1 + 1; i < nd; ++i) Q *= D[i];
+  int64_t totalX = 1, totalY = 1;
+  for (int i = 0; i < nd; ++i) {
+    totalX *= X[i];
+    totalY *= Y[i];
+  }
+  if (P * std::max(n, h) * Q >= (int64_t(1) << 31))
+    return fail(FFTAHIP_ERR_CAPACITY, "real transform needs more than 2^31 - 1 elements");
+  const bool even = n % 2 == 0;
+  const int64_t m = even ? n / 2 : n;
+  p->m = m;
+  p->lp1 = std::make_unique<LengthPlan>();
+  fftahip_status st = init_length(*p->lp1, precision, m, bluestein_cutoff);
+  if (st) return st;
+  hipStream_t s = nullptr;
+  if ((st = prepare_length(*p->lp1, p->tables, s))) return st;
+  if (d2 >= 0) {
+    p->lp2 = std::make_unique<LengthPlan>();
+    if ((st = init_length(*p->lp2, precision, D[d2], bluestein_cutoff))) return st;
+    if ((st = prepare_length(*p->lp2, p->tables, s))) return st;
+  }
+  if ((st = two_level(p->tables, n, &p->lo, &p->hi, &p->sh, s))) return st;
+
+  // scratch: [C: 2-D brfft copy of x][A: packed / extended lines][B: FFT out][schedule scratch]
+  int64_t off = 0;
+  if (direction == 1 && d2 >= 0) {
+    p->offC = off;
+    off += totalX;
+  }
+  p->offA = off;
+  off += P * m * Q;
+  p->offB = off;
+  off += P * m * Q;
+  const int strided1 = P > 1 ? 1 : 0;
+  int64_t per1;
+  int nbuf1;
+  if ((st = scratch_per_line(*p->lp1, strided1, &per1, &nbuf1))) return st;
+  const int64_t s1off = off;
+  off += per1 * P * Q * nbuf1;
+  const int64_t s2off = off;
+  int64_t s2second = off;  // second scratch buffer of the d2 schedule
+  if (d2 >= 0) {
+    LineSet l2 = lines_along(nd, direction == -1 ? Y.data() : X.data(), d2);
+    int64_t per2;
+    int nbuf2;
+    if ((st = scratch_per_line(*p->lp2, l2.es_in != 1 ? 1 : 0, &per2, &nbuf2))) return st;
+    s2second = off + per2 * l2.count();
+    off += per2 * l2.count() * nbuf2;
+  }
+  if ((st = p->scratch.ensure((size_t)std::max<int64_t>(off, 1) * csize(precision)))) return st;
+  const BufRef SA{BUF_SCRATCH, p->offA}, SB{BUF_SCRATCH, p->offB}, SC{BUF_SCRATCH, p->offC};
+  const BufRef s1a{BUF_SCRATCH, s1off}, s1b{BUF_SCRATCH, s1off + per1 * P * Q};
+
+  if (direction == -1) {
+    // ---- rfft along d1 (real X -> complex Y with h along d1)
+    if (even) {
+      BufRef src = SA;
+      LineSet ls;
+      if (P == 1) {
+        // pairs along a contiguous line are already complex elements
+        src = BufRef{BUF_IN, 0};
+        ls = line_set(1, Q, 1, m, 1, h);
+      } else {
+        p->pre_op = REAL_PACK;
+        p->pre = real_geom(P, m, Q, P, P * n, P, P * m);
+        ls = line_set(P, Q, P, P * m, P, P * h);
+      }
+      if ((st = build_launches(*p->lp1, p->tables, ls, src, BufRef{BUF_OUT, 0}, s1a, s1b, false, p->S1.L, s)))
+        return st;
+      p->post_op = REAL_RFFT_SPLIT;
+      p->post = real_geom(P, m / 2 + 1, Q, P, P * h, P, P * h);
+    } else {
+      p->pre_op = REAL_COMPLEXIFY;
+      p->pre = real_geom(P, n, Q, P, P * n, P, P * n);
+      LineSet ls = line_set(P, Q, P, P * n, P, P * n);
+      if ((st = build_launches(*p->lp1, p->tables, ls, SA, SB, s1a, s1b, false, p->S1.L, s))) return st;
+      p->post_op = REAL_COPY;
+      p->post = real_geom(P, h, Q, P, P * n, P, P * h);
+    }
+    if (d2 >= 0) {
+      LineSet l2 = lines_along(nd, Y.data(), d2);
+      if ((st = build_launches(*p->lp2, p->tables, l2, BufRef{BUF_OUT, 0}, BufRef{BUF_OUT, 0},
+                               BufRef{BUF_SCRATCH, s2off}, BufRef{BUF_SCRATCH, s2second}, false, p->S2.L, s)))
+        return st;
+    }
+  } else {
+    // ---- brfft: [complex backward along d2 into C], then along d1 (complex h -> real n)
+    if (d2 >= 0) {
+      LineSet l2 = lines_along(nd, X.data(), d2);
+      if ((st = build_launches(*p->lp2, p->tables, l2, BufRef{BUF_IN, 0}, SC, BufRef{BUF_SCRATCH, s2off},
+                               BufRef{BUF_SCRATCH, s2second}, true, p->S2.L, s)))
+        return st;
+      p->s1_from_c = true;
+    }
+    if (even) {
+      p->pre_op = REAL_BRFFT_SPLIT;
+      p->pre = real_geom(P, m / 2 + 1, Q, P, P * h, P, P * m);
+      if (P == 1) {
+        LineSet ls = line_set(1, Q, 1, m, 1, m);  // y viewed as complex pairs
+        if ((st = build_launches(*p->lp1, p->tables, ls, SA, BufRef{BUF_OUT, 0}, s1a, s1b, true, p->S1.L, s)))
+          return st;
+      } else {
+        LineSet ls = line_set(P, Q, P, P * m, P, P * m);
+        if ((st = build_launches(*p->lp1, p->tables, ls, SA, SB, s1a, s1b, true, p->S1.L, s))) return st;
+        p->post_op = REAL_UNPACK;
+        p->post = real_geom(P, m, Q, P, P * m, P, P * n);
+      }
+    } else {
+      p->pre_op = REAL_HERMITIAN;
+      p->pre = real_geom(P, n, Q, P, P * h, P, P * n);
+      LineSet ls = line_set(P, Q, P, P * n, P, P * n);
+      if ((st = build_launches(*p->lp1, p->tables, ls, SA, SB, s1a, s1b, true, p->S1.L, s))) return st;
+      p->post_op = REAL_PART;
+      p->post = real_geom(P, n, Q, P, P * n, P, P * n);
+    }
+  }
+  if ((st = upload_schedule(p->S1))) return st;
+  if ((st = upload_schedule(p->S2))) return st;
+  *plan = p.release();
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_rplan_output_dims(fftahip_rplan plan, int* ndims_out, int64_t* dims_out) {
+  if (!plan || !ndims_out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  *ndims_out = (int)plan->out_dims.size();
+  if (dims_out)
+    for (size_t i = 0; i < plan->out_dims.size(); ++i) dims_out[i] = plan->out_dims[i];
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_rplan_execute(fftahip_rplan plan, const void* d_in, void* d_out, void* stream) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  for (int64_t v : plan->in_dims)
+    if (v == 0) return FFTAHIP_SUCCESS;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t cs = csize(plan->prec);
+  char* scr = (char*)plan->scratch.p;
+  const BufPtrs bpx{d_in, d_out, plan->scratch.p};
+  fftahip_status st;
+  // 2-D brfft: x along d2 first, into C
+  if (plan->dir == 1 && plan->d2 >= 0) {
+    if ((st = run_schedule(plan->S2, plan->prec, bpx, s))) return st;
+  }
+  const void* x1 = plan->s1_from_c ? (const void*)(scr + plan->offC * cs) : d_in;
+  void* A = scr + plan->offA * cs;
+  void* B = scr + plan->offB * cs;
+  if (plan->pre_op >= 0) {
+    HIP_TRY(launch_real_boundary(plan->pre_op, plan->prec, A, x1, plan->pre, plan->m, plan->lo, plan->hi, plan->sh,
+                                 plan->n, s));
+  }
+  if ((st = run_schedule(plan->S1, plan->prec, BufPtrs{x1, d_out, plan->scratch.p}, s))) return st;
+  if (plan->post_op == REAL_RFFT_SPLIT) {
+    HIP_TRY(launch_real_boundary(REAL_RFFT_SPLIT, plan->prec, d_out, nullptr, plan->post, plan->m, plan->lo,
+                                 plan->hi, plan->sh, plan->n, s));
+  } else if (plan->post_op >= 0) {
+    HIP_TRY(launch_real_boundary(plan->post_op, plan->prec, d_out, B, plan->post, plan->m, plan->lo, plan->hi,
+                                 plan->sh, plan->n, s));
+  }
+  // 2-D rfft: then along d2, in place in y
+  if (plan->dir == -1 && plan->d2 >= 0) {
+    if ((st = run_schedule(plan->S2, plan->prec, BufPtrs{d_out, d_out, plan->scratch.p}, s))) return st;
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_rplan_execute_host(fftahip_rplan plan, const void* h_in, void* h_out) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  int64_t ti = 1, to = 1;
+  for (int64_t v : plan->in_dims) ti *= v;
+  for (int64_t v : plan->out_dims) to *= v;
+  const size_t cs = csize(plan->prec), rs = cs / 2;
+  const size_t bi = (size_t)ti * (plan->dir == -1 ? rs : cs), bo = (size_t)to * (plan->dir == -1 ? cs : rs);
+  if (bi == 0 || bo == 0) return FFTAHIP_SUCCESS;
+  fftahip_status st;
+  if ((st = plan->hin.ensure(bi))) return st;
+  if ((st = plan->hout.ensure(bo))) return st;
+  HIP_TRY(hipMemcpy(plan->hin.p, h_in, bi, hipMemcpyHostToDevice));
+  if ((st = fftahip_rplan_execute(plan, plan->hin.p, plan->hout.p, nullptr))) return st;
+  HIP_TRY(hipMemcpy(h_out, plan->hout.p, bo, hipMemcpyDeviceToHost));
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_rplan_destroy(fftahip_rplan plan) {
+  delete plan;
+  return FFTAHIP_SUCCESS;
+}
+
 static void describe_length(std::ostringstream& os, LengthPlan& lp) {
   os << "{\"n\": " << lp.N << ", \"ffta\": [";
   for (size_t i = 0; i < lp.ffta.size(); ++i) {

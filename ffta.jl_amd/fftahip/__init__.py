@@ -14,6 +14,8 @@ Reference correspondence:
   plan_ifft / ifft          AbstractFFTs ScaledPlan: bfft with 1/prod(size(x, region))
                             applied by the pass that writes the result
   fft / bfft                AbstractFFTs convenience front-ends (plan, then apply)
+  plan_rfft / plan_brfft    real transforms, src/plan.jl:102-150, 386-583
+  rfft / brfft / irfft
 Errors map to DimensionMismatch, ArgumentError and DomainError with the
 reference's messages.
 """
@@ -31,6 +33,12 @@ from .plan import (  # noqa: F401
     plan_fft,
     plan_bfft,
     plan_ifft,
+    RPlan,
+    plan_rfft,
+    plan_brfft,
+    rfft,
+    brfft,
+    irfft,
     fft,
     bfft,
     ifft,

@@ -43,6 +43,12 @@ SIGNATURES = {
     "fftahip_execute_host": (_st, [_vp, ctypes.c_int, _i64p, _vp, _vp]),
     "fftahip_plan_describe": (_st, [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "fftahip_plan_destroy": (_st, [_vp]),
+    "fftahip_rplan_create": (_st, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p,
+                                   ctypes.c_int64, ctypes.c_int, _i64p, ctypes.c_int64]),
+    "fftahip_rplan_output_dims": (_st, [_vp, ctypes.POINTER(ctypes.c_int), _i64p]),
+    "fftahip_rplan_execute": (_st, [_vp, _vp, _vp, _vp]),
+    "fftahip_rplan_execute_host": (_st, [_vp, _vp, _vp]),
+    "fftahip_rplan_destroy": (_st, [_vp]),
     "fftahip_batch_create": (_st, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int64, _i64p, _i64p,
                                    ctypes.POINTER(ctypes.c_int), ctypes.c_int64]),
     "fftahip_batch_execute": (_st, [_vp, _vp, _vp, _vp]),

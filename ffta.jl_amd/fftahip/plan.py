@@ -251,6 +251,100 @@ def ifft(x, region=None, **kw) -> np.ndarray:
     return plan_ifft(a, region, **kw) * a
 
 
+class RPlan:
+    """FFTAPlan_re: plan_rfft (direction -1, real input of `shape`) or
+    plan_brfft (direction +1, complex input of `shape`, real length `length`
+    along the first region dimension), src/plan.jl:102-150."""
+
+    def __init__(self, shape, precision: int, region, direction: int, length: int | None = None,
+                 cutoff: int = DEFAULT_BLUESTEIN_CUTOFF):
+        self._h = ctypes.c_void_p()
+        self.shape = tuple(int(d) for d in shape)
+        self.precision = precision
+        self.direction = direction
+        reg = _region_tuple(region, len(self.shape))
+        self.region = list(reg)
+        length = 0 if length is None else int(length)
+        L = load()
+        _check(L.fftahip_rplan_create(ctypes.byref(self._h), precision, direction, len(self.shape),
+                                      i64_array(self.shape), length, len(reg), i64_array(reg), cutoff))
+        nd = ctypes.c_int()
+        _check(L.fftahip_rplan_output_dims(self._h, ctypes.byref(nd), None))
+        dims = (ctypes.c_int64 * max(1, nd.value))()
+        _check(L.fftahip_rplan_output_dims(self._h, ctypes.byref(nd), dims))
+        self.output_shape = tuple(int(dims[i]) for i in range(nd.value))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.fftahip_rplan_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def execute_device(self, in_ptr: int, out_ptr: int, stream: int = 0):
+        _check(load().fftahip_rplan_execute(self._h, ctypes.c_void_p(int(in_ptr)), ctypes.c_void_p(int(out_ptr)),
+                                            ctypes.c_void_p(int(stream))))
+
+    def __mul__(self, x):
+        """p * x on a host array (staged through device memory)."""
+        real_t = np.float32 if self.precision == 32 else np.float64
+        cplx_t = _dtype_of(self.precision)
+        want = real_t if self.direction == -1 else cplx_t
+        a = np.asarray(x)
+        if a.dtype != want:
+            raise ArgumentError(f"plan element type {np.dtype(want)} does not match array element type {a.dtype}")
+        if tuple(a.shape) + (1,) * (len(self.shape) - a.ndim) != self.shape:
+            raise DimensionMismatch("plan and input array axes do not match")
+        a = np.asfortranarray(a)
+        out_t = cplx_t if self.direction == -1 else real_t
+        y = np.empty(self.output_shape, dtype=out_t, order="F")
+        _check(load().fftahip_rplan_execute_host(self._h, a.ctypes.data_as(ctypes.c_void_p),
+                                                 y.ctypes.data_as(ctypes.c_void_p)))
+        return y
+
+
+def plan_rfft(x, region=None, BLUESTEIN_CUTOFF: int = DEFAULT_BLUESTEIN_CUTOFF, **_kw) -> RPlan:
+    """AbstractFFTs.plan_rfft(x, region) (src/plan.jl:102-125): x real (Float32/Float64)."""
+    a = np.asarray(x)
+    if a.dtype not in (np.float32, np.float64):
+        raise ArgumentError(f"rfft needs a real array, got {a.dtype}")
+    return RPlan(a.shape, 32 if a.dtype == np.float32 else 64, region, FFT_FORWARD, None, BLUESTEIN_CUTOFF)
+
+
+def plan_brfft(x, length: int, region=None, BLUESTEIN_CUTOFF: int = DEFAULT_BLUESTEIN_CUTOFF, **_kw) -> RPlan:
+    """AbstractFFTs.plan_brfft(x, len, region) (src/plan.jl:127-150): x complex, result real with
+    `length` entries along the first region dimension."""
+    a = np.asarray(x)
+    return RPlan(a.shape, _precision_of(a.dtype), region, FFT_BACKWARD, length, BLUESTEIN_CUTOFF)
+
+
+def rfft(x, region=None, **kw) -> np.ndarray:
+    a = np.asarray(x)
+    if a.dtype not in (np.float32, np.float64):
+        a = a.astype(np.float64)
+    return plan_rfft(a, region, **kw) * a
+
+
+def brfft(x, length: int, region=None, **kw) -> np.ndarray:
+    a = np.asarray(x)
+    return plan_brfft(a, length, region, **kw) * a
+
+
+def irfft(x, length: int, region=None, **kw) -> np.ndarray:
+    """AbstractFFTs irfft = brfft scaled by 1/(length * prod(other region sizes))."""
+    a = np.asarray(x)
+    reg = _region_tuple(region, a.ndim)
+    n = int(length)
+    for r in reg[1:]:
+        n *= a.shape[r - 1] if r <= a.ndim else 1
+    y = brfft(a, length, region, **kw)
+    y *= 1.0 / n
+    return y
+
+
 class Batch:
     """Many 1-D transforms of uneven lengths in one work-list launch per pass."""
 

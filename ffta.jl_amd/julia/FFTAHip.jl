@@ -114,6 +114,60 @@ function Base.:*(p::FFTAHipPlan{T}, x::AbstractArray{T}) where {T}
     return y
 end
 
+# ---------------------------------------------------------------- real transforms
+# plan_rfft / plan_brfft (src/plan.jl:102-150) and p * x (src/plan.jl:386-583):
+# one library plan per input shape (FFTA's real plans are applied to the
+# array they were made from, slice by slice).
+
+mutable struct FFTAHipRPlan{T,N} <: AbstractFFTs.Plan{T}
+    handle::Ptr{Cvoid}
+    region::Any
+    dir::Cint
+    flen::Int
+    in_size::NTuple{N,Int}
+    out_size::NTuple{N,Int}
+end
+
+function _rplan(::Type{T}, sz::NTuple{N,Int}, region, dir::Cint, len::Int, cutoff::Int) where {T,N}
+    reg = Int64[region...]
+    dims = Int64[sz...]
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    prec = real(T) == Float32 ? Cint(32) : Cint(64)
+    check(ccall((:fftahip_rplan_create, libfftahip), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint, Ptr{Int64}, Int64, Cint, Ptr{Int64}, Int64),
+                h, prec, dir, N, dims, len, length(reg), reg, cutoff))
+    nd = Ref{Cint}(0)
+    out = zeros(Int64, max(N, maximum(reg)))
+    check(ccall((:fftahip_rplan_output_dims, libfftahip), Cint, (Ptr{Cvoid}, Ptr{Cint}, Ptr{Int64}), h[], nd, out))
+    p = FFTAHipRPlan{T,N}(h[], region, dir, dir == FFT_FORWARD ? sz[first(region)] : len, sz,
+                          ntuple(i -> Int(out[i]), Val(N)))
+    finalizer(p) do q
+        q.handle == C_NULL || ccall((:fftahip_rplan_destroy, libfftahip), Cint, (Ptr{Cvoid},), q.handle)
+        q.handle = C_NULL
+    end
+    return p
+end
+
+AbstractFFTs.plan_rfft(x::AbstractArray{T,N}, region; BLUESTEIN_CUTOFF=DEFAULT_BLUESTEIN_CUTOFF,
+                       _kwargs...) where {T<:Union{Float32,Float64},N} =
+    _rplan(T, size(x), sort(collect(region)), FFT_FORWARD, 0, BLUESTEIN_CUTOFF)
+
+AbstractFFTs.plan_brfft(x::AbstractArray{T,N}, len::Int, region; BLUESTEIN_CUTOFF=DEFAULT_BLUESTEIN_CUTOFF,
+                        _kwargs...) where {T<:Union{ComplexF32,ComplexF64},N} =
+    _rplan(T, size(x), sort(collect(region)), FFT_BACKWARD, len, BLUESTEIN_CUTOFF)
+
+function Base.:*(p::FFTAHipRPlan{T,N}, x::AbstractArray{T,N}) where {T,N}
+    Base.require_one_based_indexing(x)
+    size(x) == p.in_size || throw(DimensionMismatch("plan and input array axes do not match"))
+    y = p.dir == FFT_FORWARD ? Array{complex(T)}(undef, p.out_size) : Array{real(T)}(undef, p.out_size)
+    xs = x isa StridedArray ? x : collect(x)
+    GC.@preserve xs y begin
+        check(ccall((:fftahip_rplan_execute_host, libfftahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                    p.handle, pointer(xs), pointer(y)))
+    end
+    return y
+end
+
 """
     fft_batch!(y, x, lengths; directions) -> y
 

@@ -109,6 +109,30 @@ fftahip_status fftahip_plan_describe(fftahip_plan plan, char* buf, size_t cap, s
 
 fftahip_status fftahip_plan_destroy(fftahip_plan plan);
 
+/* ---------------------------------------------------------------- real transforms
+ * plan_rfft(x, region) (src/plan.jl:102-125) for a real array x of shape
+ * dims (direction -1; the result has size(x, region[1]) / 2 + 1 along the
+ * first region dimension), or plan_brfft(x, len, region)
+ * (src/plan.jl:127-150) for a complex array x of shape dims whose first
+ * region dimension has len / 2 + 1 entries (direction +1; the result is real
+ * with len entries there).  One or two region dimensions; more is
+ * FFTAHIP_ERR_ARGUMENT "only supports 1D and 2D FFTs".  The multiplication
+ * follows src/plan.jl:386-583 (split step for even lengths, complex
+ * transform for odd ones; 2-D: d1 then d2 forward, d2 then d1 backward).
+ * Real arrays are Float32/Float64 column-major; complex ones interleaved. */
+typedef struct fftahip_rplan_s* fftahip_rplan;
+
+fftahip_status fftahip_rplan_create(fftahip_rplan* plan, int precision, int direction, int ndims,
+                                    const int64_t* dims, int64_t len, int nregion, const int64_t* region,
+                                    int64_t bluestein_cutoff);
+/* Shape of the result array (ndims_out entries written to dims_out if non-NULL). */
+fftahip_status fftahip_rplan_output_dims(fftahip_rplan plan, int* ndims_out, int64_t* dims_out);
+/* p * x on device arrays of the plan's shapes (asynchronous on stream). */
+fftahip_status fftahip_rplan_execute(fftahip_rplan plan, const void* d_in, void* d_out, void* stream);
+/* p * x on host arrays (staged through device memory; synchronous). */
+fftahip_status fftahip_rplan_execute_host(fftahip_rplan plan, const void* h_in, void* h_out);
+fftahip_status fftahip_rplan_destroy(fftahip_rplan plan);
+
 /* ---------------------------------------------------------------- batches
  * Many independent 1-D transforms of uneven lengths (the batched use of
  * fft!(out, in, ...) per vector, src/algos.jl:5-34).  Transform i reads
