@@ -352,6 +352,8 @@ def per_call_fp64(torch, lengths, dirs, sample: int = 2000, seed: int = 3):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "line_pass_batched_uneven_c32_traffic.json")
+# radix schedules tuned on an MI355X for this workload (python bench.py --autotune)
+TUNING_DB = os.path.join(ROOT, "ffta.jl_amd", "tuning", "mi355x_batched_uneven_c32.txt")
 
 
 def measured_traffic(desc):
@@ -379,6 +381,9 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="length of the serving replay (0: skip)")
     ap.add_argument("--no-configs", action="store_true", help="skip the single-transform configurations")
     ap.add_argument("--no-percall", action="store_true", help="skip the per-call FP64 comparison")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the workload's in-LDS radix schedules on this device, write TUNING_DB and exit")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore TUNING_DB (default radix rules)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="after the steps, copy 1 GiB once (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args(argv)
@@ -389,9 +394,20 @@ def main(argv=None) -> int:
 
     import torch
 
+    from fftahip import autotune
     from fftahip.plan import Batch
 
     lengths, dirs = batched_uneven(args.count)
+    if args.autotune:
+        t0 = time.perf_counter()
+        res = autotune.tune_batch(lengths, dirs, 32)
+        autotune.save(TUNING_DB)
+        print(json.dumps({"autotune": res, "seconds": round(time.perf_counter() - t0, 1), "db": TUNING_DB}),
+              flush=True)
+        return 0
+    tuned_entries = 0
+    if not args.no_tuning and os.path.exists(TUNING_DB):
+        tuned_entries = autotune.load_table(TUNING_DB)
     t0 = time.perf_counter()
     batch = Batch(lengths, dirs, precision=32)
     plan_s = time.perf_counter() - t0
@@ -453,6 +469,7 @@ def main(argv=None) -> int:
             "distinct_lengths": int(np.unique(lengths).size),
             "launches_per_step": len(desc.get("phases", [])),
             "graph": not args.no_graph,
+            "autotuned_radix_schedules": tuned_entries,
         },
         "roofline": {
             "bound": "hbm",

@@ -29,6 +29,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -213,10 +214,40 @@ struct Inner {
   int block = 256;
 };
 
-// Radix schedule for an in-LDS line of length L (prime factors <= 13).
-static bool make_inner(int64_t L, Inner& in) {
+// In-LDS line schedule with the given radices (in stage order): threads per
+// line Tl = max over stages of ceil((L / R) / (16 / R)).
+static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in) {
+  if ((int)rad.size() > kMaxStages) return false;
+  int64_t prod = 1, Tl = 1;
+  for (int r : rad) {
+    if (r < 2 || r > kEPT) return false;
+    prod *= r;
+    const int64_t nb = kEPT / r;
+    Tl = std::max(Tl, ((L / r) + nb - 1) / nb);
+  }
+  if (prod != L || Tl > 1024) return false;
   in = Inner();
   in.L = L;
+  in.rad = rad;
+  in.Tl = (int)Tl;
+  in.block = Tl <= 256 ? 256 : (Tl <= 512 ? 512 : 1024);
+  return true;
+}
+
+// Radix schedules chosen by the autotuner, per (precision, line length).
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int64_t>, std::vector<int>> g_tuned;
+
+// Radix schedule for an in-LDS line of length L (prime factors <= 13): the
+// autotuned one for (prec, L) if there is one, else the default rule --
+// 13, 11, 7, 5 as single stages, powers of two in as few stages as possible
+// with sizes balanced (each <= 16), 9s then a 3, largest radix first.
+static bool make_inner(int64_t L, Inner& in, int prec = 0) {
+  if (prec) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find({prec, L});
+    if (it != g_tuned.end() && inner_from(L, it->second, in)) return true;
+  }
   int64_t m = L;
   int e2 = 0, e3 = 0;
   while (m % 2 == 0) { m /= 2; ++e2; }
@@ -234,18 +265,64 @@ static bool make_inner(int64_t L, Inner& in) {
   while (e3 >= 2) { rad.push_back(9); e3 -= 2; }
   if (e3) rad.push_back(3);
   std::sort(rad.begin(), rad.end(), [](int a, int b) { return a > b; });
-  if ((int)rad.size() > kMaxStages) return false;
-  int64_t Tl = 1;
-  for (int r : rad) {
-    const int64_t nb = kEPT / r;
-    const int64_t need = ((L / r) + nb - 1) / nb;
-    Tl = std::max(Tl, need);
+  return inner_from(L, rad, in);
+}
+
+// Candidate radix schedules of the autotuner for a line of length L: the
+// powers of two split into stages of <= 16 (balanced), <= 8 (balanced) or
+// greedy 16s; the powers of three as 9s (+3) or as 3s; each multiset in
+// descending and ascending order.
+static std::vector<std::vector<int>> inner_candidates(int64_t L) {
+  std::vector<std::vector<int>> out;
+  int64_t m = L;
+  int e2 = 0, e3 = 0;
+  while (m % 2 == 0) { m /= 2; ++e2; }
+  while (m % 3 == 0) { m /= 3; ++e3; }
+  std::vector<int> odd;
+  for (int p : {13, 11, 7, 5})
+    while (m % p == 0) { m /= p; odd.push_back(p); }
+  if (m != 1) return out;
+  std::vector<std::vector<int>> p2s, p3s;
+  auto balanced = [](int e, int cap_log) {
+    std::vector<int> r;
+    if (e == 0) return r;
+    const int ns = (e + cap_log - 1) / cap_log;
+    const int base = e / ns, extra = e % ns;
+    for (int i = 0; i < ns; ++i) r.push_back(1 << (base + (i < extra ? 1 : 0)));
+    return r;
+  };
+  p2s.push_back(balanced(e2, 4));
+  p2s.push_back(balanced(e2, 3));
+  {
+    std::vector<int> g;
+    int e = e2;
+    while (e >= 4) { g.push_back(16); e -= 4; }
+    if (e) g.push_back(1 << e);
+    p2s.push_back(g);
   }
-  if (Tl > 1024) return false;
-  in.rad = rad;
-  in.Tl = (int)Tl;
-  in.block = Tl <= 256 ? 256 : (Tl <= 512 ? 512 : 1024);
-  return true;
+  {
+    std::vector<int> r;
+    int e = e3;
+    while (e >= 2) { r.push_back(9); e -= 2; }
+    if (e) r.push_back(3);
+    p3s.push_back(r);
+  }
+  if (e3 >= 2 && e3 <= 6) p3s.push_back(std::vector<int>(e3, 3));
+  std::set<std::vector<int>> seen;
+  for (auto& a : p2s)
+    for (auto& b : p3s) {
+      std::vector<int> r = odd;
+      r.insert(r.end(), a.begin(), a.end());
+      r.insert(r.end(), b.begin(), b.end());
+      std::sort(r.begin(), r.end(), [](int x, int y) { return x > y; });
+      for (int order = 0; order < 2; ++order) {
+        std::vector<int> c = r;
+        if (order) std::reverse(c.begin(), c.end());
+        Inner tmp;
+        if (inner_from(L, c, tmp) && seen.insert(c).second) out.push_back(c);
+      }
+    }
+  return out;
 }
 
 static bool smooth13(int64_t n) {
@@ -486,11 +563,11 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
       return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(lp.S));
     d.F = {lp.bq};
     Inner qb;
-    if (!make_inner(lp.M, qb)) return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no line schedule for Bluestein length");
+    if (!make_inner(lp.M, qb, prec)) return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no line schedule for Bluestein length");
     d.inner = {qb};
     for (int64_t f : FS) {
       Inner in;
-      make_inner(f, in);
+      make_inner(f, in, prec);
       d.F.push_back(f);
       d.inner.push_back(in);
     }
@@ -501,7 +578,7 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
   const int64_t single_cap = strided ? pass_max(prec) : single_max(prec);
   if (target <= single_cap) {
     Inner in;
-    if (make_inner(target, in)) {
+    if (make_inner(target, in, prec)) {
       d.kind = lp.blue ? K_BLUE_SINGLE : K_SINGLE;
       d.F = {target};
       d.inner = {in};
@@ -513,8 +590,8 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
     d.kind = K_BLUE_3PASS;
     d.F = {M1, M2};
     Inner a, b;
-    make_inner(M1, a);
-    make_inner(M2, b);
+    make_inner(M1, a, prec);
+    make_inner(M2, b, prec);
     d.inner = {a, b};
     return FFTAHIP_SUCCESS;
   }
@@ -525,7 +602,7 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
   d.F = F;
   for (int64_t f : F) {
     Inner in;
-    make_inner(f, in);
+    make_inner(f, in, prec);
     d.inner.push_back(in);
   }
   return FFTAHIP_SUCCESS;
@@ -2365,6 +2442,141 @@ fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const 
   }
   os << "], \"phases\": " << b->phases.size() << "}";
   return emit(os.str(), buf, cap, needed);
+}
+
+// ------------------------------------------------------------------ autotuner
+
+fftahip_status fftahip_autotune_lines(int precision, int64_t count, const int64_t* lengths, int64_t* changed_out) {
+  if (precision != 32 && precision != 64) return fail(FFTAHIP_ERR_ARGUMENT, "precision must be 32 or 64");
+  if (count < 0 || (count > 0 && !lengths)) return fail(FFTAHIP_ERR_ARGUMENT, "NULL lengths");
+  const size_t esz = csize(precision);
+  // 16 MB per buffer: the passes run from the last-level cache, so the timing
+  // ranks the candidates by their instruction and LDS cost (the part the
+  // radix schedule changes) rather than by HBM bandwidth
+  const int64_t elems = precision == 32 ? (int64_t(1) << 21) : (int64_t(1) << 20);
+  DevBuf a, b;
+  fftahip_status st;
+  if ((st = a.ensure((size_t)elems * esz))) return st;
+  if ((st = b.ensure((size_t)elems * esz))) return st;
+  HIP_TRY(hipMemset(a.p, 0, (size_t)elems * esz));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int64_t changed = 0;
+  std::set<int64_t> done;
+  st = FFTAHIP_SUCCESS;
+  for (int64_t i = 0; i < count && st == FFTAHIP_SUCCESS; ++i) {
+    const int64_t L = lengths[i];
+    if (L < 2 || L > single_max(precision) || !smooth13(L) || !done.insert(L).second) continue;
+    std::vector<std::vector<int>> cands = inner_candidates(L);
+    if (cands.size() < 2) continue;
+    Inner dflt;
+    make_inner(L, dflt);
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      g_tuned.erase({precision, L});
+    }
+    double best_ms = 1e30;
+    std::vector<int> best;
+    for (const auto& c : cands) {
+      {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tuned[{precision, L}] = c;
+      }
+      const int64_t dims[2] = {L, std::max<int64_t>(1, elems / L)};
+      const int64_t region[1] = {1};
+      fftahip_plan pl = nullptr;
+      if ((st = fftahip_plan_create(&pl, precision, -1, 2, dims, 1, region, 73))) break;
+      float ms = 0;
+      st = fftahip_execute(pl, 2, dims, a.p, b.p, nullptr);
+      if (st == FFTAHIP_SUCCESS) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+        int reps = 0;
+        for (; e == hipSuccess && reps < 20; ++reps)
+          if ((st = fftahip_execute(pl, 2, dims, a.p, b.p, nullptr))) break;
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess && st == FFTAHIP_SUCCESS) st = fail(FFTAHIP_ERR_HIP, hipGetErrorString(e));
+        ms /= std::max(reps, 1);
+      }
+      fftahip_plan_destroy(pl);
+      if (st) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    if (st || best.empty() || best == dflt.rad) {
+      g_tuned.erase({precision, L});
+    } else {
+      g_tuned[{precision, L}] = best;
+      ++changed;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (changed_out) *changed_out = changed;
+  return st;
+}
+
+fftahip_status fftahip_autotune_save(const char* path) {
+  if (!path) return fail(FFTAHIP_ERR_ARGUMENT, "NULL path");
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(FFTAHIP_ERR_ARGUMENT, std::string("cannot write ") + path);
+  std::fprintf(f, "# fftahip radix schedules: precision line-length radices (stage order)\n");
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  for (const auto& kv : g_tuned) {
+    std::fprintf(f, "%d %lld", kv.first.first, (long long)kv.first.second);
+    for (int r : kv.second) std::fprintf(f, " %d", r);
+    std::fprintf(f, "\n");
+  }
+  std::fclose(f);
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_autotune_load(const char* path, int64_t* entries_out) {
+  if (!path) return fail(FFTAHIP_ERR_ARGUMENT, "NULL path");
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(FFTAHIP_ERR_ARGUMENT, std::string("cannot read ") + path);
+  char line[512];
+  int64_t n = 0;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    std::istringstream is(line);
+    int prec;
+    long long L;
+    if (!(is >> prec >> L)) continue;
+    std::vector<int> rad;
+    int r;
+    while (is >> r) rad.push_back(r);
+    Inner tmp;
+    if ((prec != 32 && prec != 64) || !inner_from(L, rad, tmp)) continue;  // skip malformed entries
+    g_tuned[{prec, (int64_t)L}] = rad;
+    ++n;
+  }
+  std::fclose(f);
+  if (entries_out) *entries_out = n;
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_autotune_clear(void) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tuned.clear();
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_autotune_get(int precision, int64_t length, int* nrad, int* radices) {
+  if (!nrad) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  Inner in;
+  if (!make_inner(length, in, precision)) return fail(FFTAHIP_ERR_ARGUMENT, "no in-LDS schedule for this length");
+  *nrad = (int)in.rad.size();
+  if (radices)
+    for (size_t i = 0; i < in.rad.size(); ++i) radices[i] = in.rad[i];
+  return FFTAHIP_SUCCESS;
 }
 
 // Host evaluation of the kernels' division by a runtime constant.

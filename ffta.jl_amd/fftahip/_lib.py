@@ -68,6 +68,12 @@ SIGNATURES = {
     "fftahip_server_stats_get": (_st, [_vp, _vp]),
     "fftahip_server_reset_stats": (_st, [_vp]),
     "fftahip_server_destroy": (_st, [_vp]),
+    "fftahip_autotune_lines": (_st, [ctypes.c_int, ctypes.c_int64, _i64p, _i64p]),
+    "fftahip_autotune_save": (_st, [ctypes.c_char_p]),
+    "fftahip_autotune_load": (_st, [ctypes.c_char_p, _i64p]),
+    "fftahip_autotune_clear": (_st, []),
+    "fftahip_autotune_get": (_st, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]),
     "fftahip_malloc": (_st, [ctypes.POINTER(_vp), ctypes.c_size_t]),
     "fftahip_free": (_st, [_vp]),
     "fftahip_memcpy_htod": (_st, [_vp, _vp, ctypes.c_size_t]),

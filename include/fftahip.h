@@ -193,6 +193,21 @@ fftahip_status fftahip_server_stats_get(fftahip_server server, fftahip_server_st
 fftahip_status fftahip_server_reset_stats(fftahip_server server);
 fftahip_status fftahip_server_destroy(fftahip_server server);
 
+/* ---------------------------------------------------------------- autotuner
+ * No counterpart in the reference (FFTA picks radices by fixed rules,
+ * src/callgraph.jl:76-126).  For each in-LDS line length in `lengths`
+ * (13-smooth, up to the one-pass limit), time the candidate radix schedules
+ * on this device with a batch of lines of that length and keep the fastest
+ * for every plan created afterwards in this process; *changed_out counts the
+ * lengths whose choice differs from the default rule.  The table can be saved
+ * and loaded (text: "precision length radices..."). */
+fftahip_status fftahip_autotune_lines(int precision, int64_t count, const int64_t* lengths, int64_t* changed_out);
+fftahip_status fftahip_autotune_save(const char* path);
+fftahip_status fftahip_autotune_load(const char* path, int64_t* entries_out);
+fftahip_status fftahip_autotune_clear(void);
+/* The radix schedule (stage order) plans use for an in-LDS line of `length`. */
+fftahip_status fftahip_autotune_get(int precision, int64_t length, int* nrad, int* radices);
+
 /* ---------------------------------------------------------------- device helpers
  * Minimal memory helpers so that a host language without a GPU array type
  * (the Julia wrapper, plain C) can run the device entry points. */

@@ -210,3 +210,33 @@ def test_register_passes(n, prec):
     assert any(la.get("reg") for la in s["launches"]), [(la["L"], la["col_in"], la["col_out"]) for la in s["launches"]]
     _check_schedule((n,), prec, -1)
     _check_schedule((n,), prec, 1)
+
+
+def test_autotune_table(tmp_path):
+    """The autotuner's table: loaded entries replace the default radix rule for
+    their (precision, length), malformed ones are skipped, plans built
+    afterwards use them (and still transform exactly), save/load round-trips."""
+    from fftahip import autotune
+
+    autotune.clear()
+    try:
+        assert autotune.schedule(32, 2025) == [9, 9, 5, 5]
+        src = tmp_path / "t.txt"
+        src.write_text("# precision length radices\n32 2025 5 5 9 9\n32 4096 8 8 8 8\n64 100 7 7\n32 12 16\n"
+                       "32 4096 32 128\n")
+        assert autotune.load_table(str(src)) == 2
+        assert autotune.schedule(32, 2025) == [5, 5, 9, 9]
+        assert autotune.schedule(64, 2025) == [9, 9, 5, 5]
+        assert autotune.schedule(32, 4096) == [8, 8, 8, 8]
+        s = P.debug_schedule((2025,), 32, -1)
+        assert s["launches"][0]["rad"] == [5, 5, 9, 9]
+        _check_schedule((2025,), 32, -1)
+        _check_schedule((4096, 3), 32, 1)
+        out = tmp_path / "o.txt"
+        autotune.save(str(out))
+        autotune.clear()
+        assert autotune.schedule(32, 4096) == [16, 16, 16]
+        assert autotune.load_table(str(out)) == 2
+        assert autotune.schedule(32, 4096) == [8, 8, 8, 8]
+    finally:
+        autotune.clear()
