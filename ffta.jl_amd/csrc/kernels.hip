@@ -77,8 +77,10 @@ __device__ __forceinline__ double2 unit_root(int64_t m, int64_t n) {
 
 // Padded position of element i inside an LDS line (one spare slot every
 // 2^sh elements breaks the power-of-two strides of the radix stages across
-// the LDS banks).
-__device__ __forceinline__ int pad_idx(int i, int sh) { return sh ? i + (i >> sh) : i; }
+// the LDS banks).  Takes the effective shift she = sh ? sh : 31, so that no
+// padding needs no select (i < 2^31).
+__device__ __forceinline__ int pad_shift(int sh) { return sh ? sh : 31; }
+__device__ __forceinline__ int pad_idx(int i, int she) { return i + (i >> she); }
 
 // One radix-RAD Stockham stage over a line in LDS.  Stage twiddles come from
 // a per-stage table: stage s (Ns = product of earlier radices) keeps
@@ -93,6 +95,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
                                       const StF& st) {
   constexpr int NB = EPT / RAD;
   const int Lr = L / RAD;
+  const int she = pad_shift(sh);
   const int pmask = sh ? (1 << sh) - 1 : 0;
   const bool rstep = (Lr & pmask) == 0;
   const bool wstep = (Ns & pmask) == 0;
@@ -122,12 +125,12 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
 #pragma unroll
         for (int t = 0; t < RAD; ++t) v[b][t] = ld(j + t * Lr);
       } else if (rstep) {
-        const int p0 = pad_idx(j, sh);
+        const int p0 = pad_idx(j, she);
 #pragma unroll
         for (int t = 0; t < RAD; ++t) v[b][t] = line[p0 + t * rstride];
       } else {
 #pragma unroll
-        for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, sh)];
+        for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, she)];
       }
       if (Ns > 1) {
 #pragma unroll
@@ -148,12 +151,12 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
 #pragma unroll
         for (int t = 0; t < RAD; ++t) st(base + t * Ns, v[b][t]);
       } else if (wstep) {
-        const int p0 = pad_idx(base, sh);
+        const int p0 = pad_idx(base, she);
 #pragma unroll
         for (int t = 0; t < RAD; ++t) line[p0 + t * wstride] = v[b][t];
       } else {
 #pragma unroll
-        for (int t = 0; t < RAD; ++t) line[pad_idx(base + t * Ns, sh)] = v[b][t];
+        for (int t = 0; t < RAD; ++t) line[pad_idx(base + t * Ns, she)] = v[b][t];
       }
     }
   }
@@ -316,6 +319,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const int L = p.L;
   const int ldl = p.ldl;
   const int psh = p.lds_sh;
+  const int pshe = pad_shift(psh);
   const int64_t line0 = (int64_t)tile * C;
   const int64_t rem = p.nlines - line0;
   const int nvalid = rem < C ? (int)rem : C;
@@ -411,7 +415,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
               x = cmul(x, pass_twiddle_value<Rt>(pw[u]));
             }
           }
-          lds[c * ldl + pad_idx(t, psh)] = x;
+          lds[c * ldl + pad_idx(t, pshe)] = x;
         }
       }
     }
@@ -431,12 +435,15 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         if (p.col_in) {
           t = (int)fdiv((uint32_t)e, p.divC);
           c = e - t * C;
+        } else if (C == 1) {
+          c = 0;
+          t = e;
         } else {
           c = (int)fdiv((uint32_t)e, p.divL);
           t = e - c * L;
         }
         if (c < nvalid) {
-          slot[u] = c * ldl + pad_idx(t, psh);
+          slot[u] = c * ldl + pad_idx(t, pshe);
           bool live = true;
           if (chirp_in) {
             const int64_t n = sh.cin[c] + (int64_t)t * p.cae;
@@ -509,10 +516,10 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         const int e = e0 + u * BLOCK + tid;
         slot[u] = -1;
         if (e < total) {
-          const int c = (int)fdiv((uint32_t)e, p.divL);
+          const int c = C == 1 ? 0 : (int)fdiv((uint32_t)e, p.divL);
           const int t = e - c * L;
           if (c < nvalid) {
-            slot[u] = c * ldl + pad_idx(t, psh);
+            slot[u] = c * ldl + pad_idx(t, pshe);
             f[u] = gld(bhat + sh.cmid[c] + (int64_t)t * p.cme);
           }
         }
@@ -533,7 +540,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         cx<Rt>* __restrict__ dst = out + sh.out[my_c];
         const cx<Rt>* line = lds + my_c * ldl;
         for (int t = my_t0; t < L; t += tstep) {
-          cx<Rt> y = cconj_if(line[pad_idx(t, psh)], sgn_out);
+          cx<Rt> y = cconj_if(line[pad_idx(t, pshe)], sgn_out);
           if (has_scale) y = cscale(y, scale);
           gst(dst + (int64_t)t * p.be, y);
         }
@@ -549,12 +556,15 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         if (p.col_out) {
           t = (int)fdiv((uint32_t)e, p.divC);
           c = e - t * C;
+        } else if (C == 1) {
+          c = 0;
+          t = e;
         } else {
           c = (int)fdiv((uint32_t)e, p.divL);
           t = e - c * L;
         }
         if (c >= nvalid) continue;
-        cx<Rt> y = cconj_if(lds[c * ldl + pad_idx(t, psh)], sgn_out);
+        cx<Rt> y = cconj_if(lds[c * ldl + pad_idx(t, pshe)], sgn_out);
         if (has_scale) y = cscale(y, scale);
         gst(out + sh.out[c] + (int64_t)t * p.be, y);
       }
@@ -591,7 +601,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
             }
           }
           if (!kGeo && pretw_out) pw[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
-          x[u] = lds[c * ldl + pad_idx(t, psh)];
+          x[u] = lds[c * ldl + pad_idx(t, pshe)];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -630,6 +640,9 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
       if (p.col_out) {
         t = (int)fdiv((uint32_t)e, p.divC);
         c = e - t * C;
+      } else if (C == 1) {
+        c = 0;
+        t = e;
       } else {
         c = (int)fdiv((uint32_t)e, p.divL);
         t = e - c * L;
@@ -645,7 +658,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         }
       }
       if (pretw_out) w[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
-      x[u] = lds[c * ldl + pad_idx(t, psh)];
+      x[u] = lds[c * ldl + pad_idx(t, pshe)];
       o[u] = sh.out[c] + (int64_t)t * p.be;
       ok[u] = true;
     }
