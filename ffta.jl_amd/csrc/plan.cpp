@@ -1368,6 +1368,7 @@ struct fftahip_batch_s {
     int64_t grid;
     int ngroups;
     size_t pass_off, tile_off;  // element offsets into passes / tiles
+    std::vector<int> deps;      // earlier launches holding the previous pass of one of its groups
   };
   std::vector<Phase> phases;
   std::vector<LinePass> host_passes;
@@ -1375,8 +1376,15 @@ struct fftahip_batch_s {
   hipGraphExec_t graph = nullptr;
   const void* graph_in = nullptr;
   void* graph_out = nullptr;
+  // launches run on kStreams streams, ordered only by their dependencies
+  static constexpr int kStreams = 4;
+  hipStream_t streams[kStreams] = {};
+  std::vector<hipEvent_t> events;  // one per launch, then fork, then one join per stream
   ~fftahip_batch_s() {
     if (graph) (void)hipGraphExecDestroy(graph);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    for (hipStream_t q : streams)
+      if (q) (void)hipStreamDestroy(q);
   }
 };
 
@@ -2152,6 +2160,17 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
   };
   size_t maxp = 0;
   for (auto& L : per_group) maxp = std::max(maxp, L.size());
+  // launch_of[g][ph]: the launch that runs pass ph of group g
+  std::vector<std::vector<int>> launch_of(per_group.size());
+  for (size_t g = 0; g < per_group.size(); ++g) launch_of[g].assign(per_group[g].size(), -1);
+  auto add_deps = [&](fftahip_batch_s::Phase& P, size_t g, size_t ph) {
+    for (size_t q = ph; q-- > 0;) {
+      const int d = launch_of[g][q];
+      if (d < 0) continue;  // an empty pass: depend on the one before it
+      if (std::find(P.deps.begin(), P.deps.end(), d) == P.deps.end()) P.deps.push_back(d);
+      break;
+    }
+  };
   for (size_t ph = 0; ph < maxp; ++ph) {
     // register passes of this phase: one launch per mode
     for (int mode = 0; mode < 3; ++mode) {
@@ -2164,12 +2183,15 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
       P.pass_off = b->host_passes.size();
       P.tile_off = b->host_tiles.size();
       uint32_t tiles = 0;
-      for (auto& L : per_group) {
+      for (size_t g = 0; g < per_group.size(); ++g) {
+        auto& L = per_group[g];
         if (ph >= L.size() || !L[ph].reg || L[ph].grid <= 0 || pass_mode(L[ph].lp) != mode) continue;
         b->host_passes.push_back(L[ph].lp);
         b->host_tiles.push_back(tiles);
         tiles += (uint32_t)L[ph].grid;
         ++P.ngroups;
+        add_deps(P, g, ph);
+        launch_of[g][ph] = (int)b->phases.size();
       }
       if (P.ngroups == 0) continue;
       P.grid = tiles;
@@ -2187,7 +2209,8 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
         P.pass_off = b->host_passes.size();
         P.tile_off = b->host_tiles.size();
         uint32_t tiles = 0;
-        for (auto& L : per_group) {
+        for (size_t g = 0; g < per_group.size(); ++g) {
+          auto& L = per_group[g];
           if (ph >= L.size() || L[ph].reg || L[ph].block != block || L[ph].grid <= 0 ||
               lds_class(L[ph].lds) != cls || pass_mode(L[ph].lp) != mode)
             continue;
@@ -2198,6 +2221,8 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
           tiles += (uint32_t)L[ph].grid;
           P.lds = std::max(P.lds, L[ph].lds);
           ++P.ngroups;
+          add_deps(P, g, ph);
+          launch_of[g][ph] = (int)b->phases.size();
         }
         if (P.ngroups == 0) continue;
         P.grid = tiles;
@@ -2206,6 +2231,15 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
     }
   }
   if (g_dry) return FFTAHIP_SUCCESS;
+  // the streams and events batch_launch orders the launches with
+  const int K = fftahip_batch_s::kStreams;
+  if (!b->streams[0])
+    for (int k = 0; k < K; ++k) HIP_TRY(hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking));
+  if (b->events.size() != b->phases.size() + 1 + K) {
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    b->events.assign(b->phases.size() + 1 + K, nullptr);
+    for (auto& e : b->events) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   fftahip_status st = b->passes.ensure(b->host_passes.size() * sizeof(LinePass));
   if (st) return st;
   if ((st = b->tiles.ensure(b->host_tiles.size() * sizeof(uint32_t)))) return st;
@@ -2218,7 +2252,13 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
   return FFTAHIP_SUCCESS;
 }
 
-static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
+// Eager execution: the launches of a batch form a DAG (launch i waits only
+// for the launches that ran the previous pass of its groups), issued on
+// kStreams streams that fork from and join back into `s`; independent
+// launches (e.g. a bandwidth-bound Stockham pass and an instruction-bound
+// Bluestein pass of other lengths) overlap on the device.  Graph capture
+// uses batch_launch_serial.
+static fftahip_status batch_launch_serial(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
   const BufPtrs bp{d_in, d_out, b->scratch_ptr};
   for (auto& P : b->phases) {
     if (P.reg) {
@@ -2228,6 +2268,44 @@ static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_ou
     }
     HIP_TRY(launch_line_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
                                     P.ngroups, P.mode, b->prec, P.block, bp, 0, P.lds, P.grid, s));
+  }
+  return FFTAHIP_SUCCESS;
+}
+
+static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
+  const BufPtrs bp{d_in, d_out, b->scratch_ptr};
+  const int K = fftahip_batch_s::kStreams;
+  const size_t np = b->phases.size();
+  // streams and events are made by batch_build
+  if (!b->streams[0] || b->events.size() != np + 1 + K)
+    return fail(FFTAHIP_ERR_HIP, "batch streams/events missing");
+  hipEvent_t fork = b->events[np];
+  HIP_TRY(hipEventRecord(fork, s));
+  for (int k = 0; k < K; ++k) HIP_TRY(hipStreamWaitEvent(b->streams[k], fork, 0));
+  std::vector<int> stream_of(np, 0);
+  int rr = 0;
+  for (size_t i = 0; i < np; ++i) {
+    const auto& P = b->phases[i];
+    // a launch with a single predecessor follows it on its stream (no event
+    // wait); otherwise the streams are taken in turn
+    const int k = P.deps.size() == 1 ? stream_of[P.deps[0]] : (rr++ % K);
+    stream_of[i] = k;
+    hipStream_t q = b->streams[k];
+    for (int d : P.deps)
+      if (stream_of[d] != k) HIP_TRY(hipStreamWaitEvent(q, b->events[d], 0));
+    if (P.reg) {
+      HIP_TRY(launch_reg_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
+                                     P.ngroups, P.mode, b->prec, bp, P.grid, q));
+    } else {
+      HIP_TRY(launch_line_pass_groups((const LinePass*)b->passes.p + P.pass_off,
+                                      (const uint32_t*)b->tiles.p + P.tile_off, P.ngroups, P.mode, b->prec, P.block,
+                                      bp, 0, P.lds, P.grid, q));
+    }
+    HIP_TRY(hipEventRecord(b->events[i], q));
+  }
+  for (int k = 0; k < K; ++k) {
+    HIP_TRY(hipEventRecord(b->events[np + 1 + k], b->streams[k]));
+    HIP_TRY(hipStreamWaitEvent(s, b->events[np + 1 + k], 0));
   }
   return FFTAHIP_SUCCESS;
 }
@@ -2251,7 +2329,10 @@ fftahip_status fftahip_batch_execute_graph(fftahip_batch b, const void* d_in, vo
     HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t g;
     HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    fftahip_status st = batch_launch(b, d_in, d_out, cs);
+    // the captured graph is one chain: a multi-stream capture of the launch
+    // DAG ended the process with a host segmentation fault on this runtime
+    // (cause not found), so capture keeps the serial launch order
+    fftahip_status st = batch_launch_serial(b, d_in, d_out, cs);
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (st) return st;
