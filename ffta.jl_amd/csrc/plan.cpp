@@ -781,6 +781,13 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   if (strided) C = std::max<int64_t>(C, min_lines(prec));
   C = std::min<int64_t>(C, kMaxLines);
   C = std::min<int64_t>(C, std::max<int64_t>(1, 1024 / in.Tl));
+  // fill the workgroup size the tile needs with as many lines as it holds
+  // (e.g. 40 threads per line: 25 lines in 1024 threads, not 16 in 640)
+  if (env_i64("FFTAHIP_FILL", 1) != 0) {
+    const int64_t thr = C * in.Tl;
+    const int64_t blk = thr <= 256 ? 256 : (thr <= 512 ? 512 : 1024);
+    C = std::max<int64_t>(C, std::min<int64_t>(kMaxLines, blk / in.Tl));
+  }
   const int64_t padmax = in.L + (in.L - 1) / 16 + 1;  // largest pitch any padding choice needs
   const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pass_budget(), padmax * esz)) : lds_budget();
   while (C > 1 && C * padmax * esz > budget) --C;
