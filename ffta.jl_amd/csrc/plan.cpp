@@ -2179,8 +2179,10 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
     }
   };
   for (size_t ph = 0; ph < maxp; ++ph) {
-    // register passes of this phase: one launch per mode
-    for (int mode = 0; mode < 3; ++mode) {
+    // register passes of this phase: one launch per mode.  Within a phase
+    // the full (Bluestein) variants are issued first: they sit on the
+    // longest dependency chains of the launch DAG.
+    for (int mode = 2; mode >= 0; --mode) {
       fftahip_batch_s::Phase P;
       P.reg = true;
       P.block = kRegBlock;
@@ -2204,7 +2206,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
       P.grid = tiles;
       b->phases.push_back(P);
     }
-    for (int mode = 0; mode < 3; ++mode)
+    for (int mode = 2; mode >= 0; --mode)
     for (int block : {256, 512, 1024}) {
       for (size_t cls : kLdsClass) {
         fftahip_batch_s::Phase P;
