@@ -82,7 +82,12 @@ struct LinePass {
   int32_t lds_sh;  // LDS padding: element i of a line sits at i + (i >> lds_sh) (0: none)
   int32_t col_in, col_out;  // 1: consecutive lines are adjacent in memory (coalesce across lines)
   int32_t fuse_in, fuse_out; // 1: first stage loads from / last stage stores to global memory directly
+  // 1: the load / spectrum / store phase of a row-layout side runs stepped,
+  // one line per block/C threads (each access of a wave covers >= 128
+  // contiguous bytes of the line, or the whole line)
+  int32_t row_in, row_mid, row_out, row_pad_;
   FastDiv divC, divL;       // by C and by L
+  FastDiv divR;             // by block / C
   int64_t nlines;           // < 2^31
   int64_t n[3];
   FastDiv divn[3];          // by n[0], n[1], n[2]

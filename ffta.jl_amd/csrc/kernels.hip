@@ -359,30 +359,37 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   // FP32 passes take the pass twiddles along a thread's run of positions as
   // powers of one FP64 ratio (see reg_line); FP64 passes look each one up.
   constexpr bool kGeo = sizeof(Rt) == 4;
-  // Column layout: thread tid < C * (BLOCK / C) stays on one line
-  // (c = tid % C) and steps along it by BLOCK / C positions, so that offsets,
-  // LDS slots and pass twiddles advance incrementally instead of being
-  // re-derived; the remaining BLOCK % C threads sit out the load and store.
+  // Stepped phases: thread tid < C * (BLOCK / C) stays on one line and steps
+  // along it by tstep = BLOCK / C positions, so that offsets, LDS slots and
+  // pass twiddles advance incrementally instead of being re-derived per
+  // element; the remaining BLOCK % C threads sit out the phase.  Column layout
+  // (lines adjacent in memory): c = tid % C, neighbouring lanes on
+  // neighbouring lines.  Row layout (row_in / row_mid / row_out): c =
+  // tid / tstep, neighbouring lanes on neighbouring positions of one line.
   const int tstep = BLOCK / C;
   const bool col_step = tstep >= 1;
   const int my_t0 = (int)fdiv((uint32_t)tid, p.divC);
   const int my_c = my_t0 < tstep ? (int)(tid - (uint32_t)my_t0 * (uint32_t)C) : C;
+  const int row_c = (int)fdiv((uint32_t)tid, p.divR);  // >= C past the last line's threads
+  const int row_t0 = tid - row_c * tstep;
 
   // ---- load (global -> LDS), coalesced across lines or along a line.  All
   // global loads of a batch (element, chirp factor, pass-twiddle halves) are
   // issued before any of them is consumed.
-  if (!fuse_in && p.col_in && col_step) {
-    if (my_c < nvalid) {
-      const int c = my_c;
+  if (!fuse_in && (p.col_in || p.row_in) && col_step) {
+    const int in_c = p.col_in ? my_c : row_c;
+    const int in_t0 = p.col_in ? my_t0 : row_t0;
+    if (in_c < nvalid) {
+      const int c = in_c;
       const cx<Rt>* __restrict__ src = in + sh.in[c];
       const int64_t cin0 = MODE == MODE_FULL ? sh.cin[c] : 0;
       double2 w = make_double2(1.0, 0.0), wr = w;
       // exponents k*t*s with t < L only (the tables end at M)
-      if (kGeo && pretw_in && my_t0 < L) {
-        w = step_twiddle(p, sh.twk[c] * (int64_t)my_t0);
+      if (kGeo && pretw_in && in_t0 < L) {
+        w = step_twiddle(p, sh.twk[c] * (int64_t)in_t0);
         if (tstep < L) wr = step_twiddle(p, sh.twk[c] * (int64_t)tstep);
       }
-      for (int t0 = my_t0; t0 < L; t0 += tstep * U) {
+      for (int t0 = in_t0; t0 < L; t0 += tstep * U) {
         cx<Rt> v[U], f[U];
         PassTw<Rt> pw[U];
 #pragma unroll
@@ -508,6 +515,29 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, fuse_in, fuse_out && !has_mid, ld, st);
 
   if (has_mid) {
+    if (p.row_mid && col_step) {
+      if (row_c < nvalid) {
+        const int c = row_c;
+        const cx<Rt>* __restrict__ fsrc = bhat + sh.cmid[c];
+        cx<Rt>* line = lds + c * ldl;
+        for (int t0 = row_t0; t0 < L; t0 += tstep * U) {
+          cx<Rt> f[U], x[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * tstep;
+            if (t < L) {
+              f[u] = gld(fsrc + (int64_t)t * p.cme);
+              x[u] = line[pad_idx(t, pshe)];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * tstep;
+            if (t < L) line[pad_idx(t, pshe)] = cconj(cmul(x[u], f[u]));
+          }
+        }
+      }
+    } else
     for (int e0 = 0; e0 < total; e0 += BLOCK * U) {
       cx<Rt> f[U];
       int slot[U];
@@ -535,11 +565,13 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   // ---- store (LDS -> global).  As on the load side, the table factors of a
   // batch are all requested before the first store of the batch.
   if (MODE == MODE_PLAIN) {
-    if (!fuse_out && p.col_out && col_step) {
-      if (my_c < nvalid) {
-        cx<Rt>* __restrict__ dst = out + sh.out[my_c];
-        const cx<Rt>* line = lds + my_c * ldl;
-        for (int t = my_t0; t < L; t += tstep) {
+    if (!fuse_out && (p.col_out || p.row_out) && col_step) {
+      const int oc = p.col_out ? my_c : row_c;
+      const int ot0 = p.col_out ? my_t0 : row_t0;
+      if (oc < nvalid) {
+        cx<Rt>* __restrict__ dst = out + sh.out[oc];
+        const cx<Rt>* line = lds + oc * ldl;
+        for (int t = ot0; t < L; t += tstep) {
           cx<Rt> y = cconj_if(line[pad_idx(t, pshe)], sgn_out);
           if (has_scale) y = cscale(y, scale);
           gst(dst + (int64_t)t * p.be, y);
@@ -571,18 +603,20 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     }
     return;
   }
-  if (p.col_out && col_step) {
-    if (my_c < nvalid) {
-      const int c = my_c;
+  if ((p.col_out || p.row_out) && col_step) {
+    const int oc = p.col_out ? my_c : row_c;
+    const int ot0 = p.col_out ? my_t0 : row_t0;
+    if (oc < nvalid) {
+      const int c = oc;
       cx<Rt>* __restrict__ dst = out + sh.out[c];
       const int64_t cout0 = sh.cout[c];
       double2 w = make_double2(1.0, 0.0), wr = w;
       // exponents k*t*s with t < L only (the tables end at M)
-      if (kGeo && pretw_out && my_t0 < L) {
-        w = step_twiddle(p, sh.twk[c] * (int64_t)my_t0);
+      if (kGeo && pretw_out && ot0 < L) {
+        w = step_twiddle(p, sh.twk[c] * (int64_t)ot0);
         if (tstep < L) wr = step_twiddle(p, sh.twk[c] * (int64_t)tstep);
       }
-      for (int t0 = my_t0; t0 < L; t0 += tstep * U) {
+      for (int t0 = ot0; t0 < L; t0 += tstep * U) {
         cx<Rt> x[U], f[U];
         PassTw<Rt> pw[U];
         bool ok[U];

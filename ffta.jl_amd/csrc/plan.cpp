@@ -847,6 +847,17 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   p.divC = make_fastdiv((uint32_t)C);
   p.divL = make_fastdiv((uint32_t)in.L);
   L.block = threads <= 256 ? 256 : (threads <= 512 ? 512 : 1024);
+  // stepped row-layout phases: block/C threads per line, coalesced when they
+  // cover >= 128 bytes per access or the whole line
+  {
+    const int64_t Tr = L.block / C;
+    const bool wide = env_i64("FFTAHIP_ROWSTEP", 1) != 0 && (Tr * esz >= 128 || Tr >= in.L);
+    p.divR = make_fastdiv((uint32_t)Tr);
+    p.row_in = (wide && !p.col_in && !p.fuse_in) ? 1 : 0;
+    p.row_out = (wide && !p.col_out && !p.fuse_out) ? 1 : 0;
+    p.row_mid = wide ? 1 : 0;
+    p.row_pad_ = 0;
+  }
   L.lds = (size_t)(C * ldl * esz);
   L.grid = (p.nlines + C - 1) / C;
   p.xcd_remap = (L.grid >= 16 && L.grid % 8 == 0) ? 1 : 0;
@@ -2444,7 +2455,8 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   os << ", \"L\": " << p.L << ", \"rad\": [";
   for (int i = 0; i < p.nst; ++i) os << (i ? ", " : "") << p.rad[i];
   os << "], \"C\": " << p.C << ", \"Tl\": " << p.Tl << ", \"ldl\": " << p.ldl << ", \"lds_sh\": " << p.lds_sh << ", \"col_in\": " << p.col_in
-     << ", \"col_out\": " << p.col_out << ", \"fuse_in\": " << p.fuse_in << ", \"fuse_out\": " << p.fuse_out << ", \"nlines\": " << p.nlines << ", \"n\": [" << p.n[0] << ", " << p.n[1]
+     << ", \"col_out\": " << p.col_out << ", \"fuse_in\": " << p.fuse_in << ", \"fuse_out\": " << p.fuse_out << ", \"row_in\": " << p.row_in
+     << ", \"row_mid\": " << p.row_mid << ", \"row_out\": " << p.row_out << ", \"nlines\": " << p.nlines << ", \"n\": [" << p.n[0] << ", " << p.n[1]
      << ", " << p.n[2] << "], \"a\": [" << p.a[0] << ", " << p.a[1] << ", " << p.a[2] << ", " << p.a[3]
      << "], \"b\": [" << p.b[0] << ", " << p.b[1] << ", " << p.b[2] << ", " << p.b[3] << "], \"ae\": " << p.ae
      << ", \"be\": " << p.be << ", ";
