@@ -126,6 +126,11 @@ struct LinePass {
   int64_t cae, cbe, cme;  // spectrum index in the middle of a pass: n = sum_d i_d*cm[d] + k*cme
   const void* chirp;  // chirp_n entries, pass precision
   const void* bhat;   // spectrum factor (mid: L entries; store: indexed by n)
+  // Register passes: the chirp of a line's positions n = a + t*s is formed as
+  // c[a] * w^t * d[t] (w = exp(2 pi i a s / N)), with d[t] =
+  // exp(i pi (t^2 s^2 mod 2N) / N) from this FP64 table (L entries; null:
+  // read the chirp table instead).
+  const double2* chirp_d;
 };
 
 // Radix schedule of an in-LDS line (for table generation).

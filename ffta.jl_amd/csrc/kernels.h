@@ -28,6 +28,8 @@ enum RealOp { REAL_PACK = 0, REAL_UNPACK, REAL_COMPLEXIFY, REAL_PART, REAL_COPY,
 hipError_t launch_real_boundary(int op, int precision, void* dst, const void* src, const RealGeom& g, int64_t m,
                                 const double2* lo, const double2* hi, int sh, int64_t n, hipStream_t s);
 hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_t N, hipStream_t stream);
+// Chirp steps d[t] = exp(i pi (t^2 s^2 mod 2N) / N), t < L (FP64), of a register pass.
+hipError_t launch_gen_chirp_steps(double2* d, int64_t s, int64_t N, int L, hipStream_t stream);
 hipError_t launch_gen_chirp_ext(double2* bext, const double2* chirp64, int64_t N, int64_t M, hipStream_t stream);
 hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, hipStream_t stream);
 hipError_t launch_finish_bhat(void* bhat, const double2* spec64, int precision, int64_t M, hipStream_t stream);

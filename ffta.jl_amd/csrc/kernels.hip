@@ -43,6 +43,13 @@ __device__ __forceinline__ double2 gld(const double2* p) {
   return make_double2(v.x, v.y);
 }
 __device__ __forceinline__ int64_t gld(const int64_t* p) { return *(const FFTAHIP_GLOBAL int64_t*)p; }
+// A table entry that every lane of a wave reads at the same address, through
+// the constant address space (a scalar load into SGPRs, no VGPRs held).
+__device__ __forceinline__ double2 uld(const double2* p) {
+  typedef double V __attribute__((ext_vector_type(2)));
+  const V v = *(const __attribute__((address_space(4))) V*)p;
+  return make_double2(v.x, v.y);
+}
 template <class R>
 __device__ __forceinline__ void gst(cx<R>* p, cx<R> a) {
   typedef R V __attribute__((ext_vector_type(2)));
@@ -71,6 +78,17 @@ __device__ __forceinline__ double2 unit_root(int64_t m, int64_t n) {
   double s, c;
   sincospi(x, &s, &c);
   return make_double2(c, s);
+}
+
+// exp(i pi r / N) for the chirp exponent r = (m mod 2N) folded into (-N, N]
+// (the chirp table's own convention), m >= 0, N < 2^31.
+__device__ __forceinline__ double2 chirp_root(uint64_t m, int64_t N) {
+  const uint64_t twoN = 2 * (uint64_t)N;
+  int64_t r = (int64_t)(m % twoN);
+  if (r > N) r -= (int64_t)twoN;
+  double sn, cs;
+  sincospi((double)r / (double)N, &sn, &cs);
+  return make_double2(cs, sn);
 }
 
 // ------------------------------------------------------------------ stages
@@ -763,6 +781,37 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
   const int store_mode = MODE == MODE_FULL ? p.store_mode : STORE_PLAIN;
 
   cx<Rt> v[L];
+  // Computed chirp (chirp_d set): c[a + t*s] = c[a] * w^t * d[t], where
+  // (a + t s)^2 = a^2 + 2 a s t + s^2 t^2; c[a] and w = exp(i pi 2as / N)
+  // come from sincospi once per line (exact integer reduction), w^t as an
+  // FP64 progression (< 48 steps), d[t] from the pass's FP64 table.  No
+  // table loads per element, so every element load of the line is in flight
+  // at once.
+  if (chirp_in && p.chirp_d) {
+    const int64_t N = p.chirp_n;
+    const uint64_t a = (uint64_t)o.cin, st = (uint64_t)p.cae;
+    double2 c = chirp_root(a * a, N);
+    const double2 w = chirp_root(2 * ((a * st) % (uint64_t)N), N);
+    // element loads in chunks of kChirpChunk in flight (register budget)
+    constexpr int kChirpChunk = 8;
+#pragma unroll
+    for (int t0 = 0; t0 < L; t0 += kChirpChunk) {
+#pragma unroll
+      for (int t = t0; t < t0 + kChirpChunk && t < L; ++t) {
+        const bool live = o.cin + (int64_t)t * p.cae < N;
+        v[t] = gld(in + (live ? (int64_t)t * p.ae : 0));
+      }
+#pragma unroll
+      for (int t = t0; t < t0 + kChirpChunk && t < L; ++t) {
+        const bool live = o.cin + (int64_t)t * p.cae < N;
+        const cx<Rt> ct = from_d2<Rt>(dmul2(c, uld(p.chirp_d + t)));
+        const cx<Rt> y = cmul(cconj_if(v[t], sgn_in), cconj(ct));
+        v[t] = live ? y : cx<Rt>{(Rt)0, (Rt)0};
+        c = dmul2(c, w);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else
   // positions past the chirp length read the line's first element (always
   // in bounds) and are zeroed, so the loads stay unconditional
 #pragma unroll
@@ -804,6 +853,14 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
   const Rt scale = (Rt)p.scale;
   const bool has_scale = p.scale != 1.0;
   double2 w = wr;
+  // computed output chirp (as on the load side): c[a + k*s] = c[a] w^k d[k]
+  const bool chirp_calc = store_mode == STORE_CHIRP && p.chirp_d;
+  double2 oc = make_double2(1.0, 0.0), ow = oc;
+  if (chirp_calc) {
+    const uint64_t a = (uint64_t)o.cout, st = (uint64_t)p.cbe;
+    oc = chirp_root(a * a, p.chirp_n);
+    ow = chirp_root(2 * ((a * st) % (uint64_t)p.chirp_n), p.chirp_n);
+  }
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     cx<Rt> y = v[k];
@@ -821,7 +878,12 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
     } else if (store_mode == STORE_CHIRP) {
       const int64_t n = o.cout + (int64_t)k * p.cbe;
       live = n < p.chirp_n;
-      y = cconj(cmul(gld(chirp + (live ? n : 0)), y));
+      if (chirp_calc) {
+        y = cconj(cmul(from_d2<Rt>(dmul2(oc, uld(p.chirp_d + k))), y));
+        oc = dmul2(oc, ow);
+      } else {
+        y = cconj(cmul(gld(chirp + (live ? n : 0)), y));
+      }
     }
     y = cconj_if(y, sgn_out);
     if (has_scale) y = cscale(y, scale);
@@ -1083,6 +1145,16 @@ __global__ void gen_chirp_kernel(cx<Rt>* chirp, double2* chirp64, int64_t N) {
   if (chirp64) chirp64[n] = make_double2(c, s);
 }
 
+// Chirp steps of a register pass: d[t] = exp(i pi (t^2 s^2 mod 2N) / N),
+// t < L, FP64 (exact integer reduction: t*s mod 2N < 2^32, squared < 2^64).
+__global__ void gen_chirp_steps_kernel(double2* d, int64_t s, int64_t N, int L) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= L) return;
+  const uint64_t twoN = 2 * (uint64_t)N;
+  const uint64_t u = ((uint64_t)t * (uint64_t)s) % twoN;
+  d[t] = chirp_root(u * u, N);
+}
+
 // Periodic extension of the chirp into M slots (b[m], b[M-m] = b[m] for
 // 1 <= m < N, zeros between), FP64.
 __global__ void gen_chirp_ext_kernel(double2* bext, const double2* chirp64, int64_t N, int64_t M) {
@@ -1291,6 +1363,11 @@ hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_
   else
     hipLaunchKernelGGL(gen_chirp_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
                        reinterpret_cast<cx<double>*>(chirp), chirp64, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_chirp_steps(double2* d, int64_t s, int64_t N, int L, hipStream_t stream) {
+  hipLaunchKernelGGL(gen_chirp_steps_kernel, dim3((uint32_t)((L + 63) / 64)), dim3(64), 0, stream, d, s, N, L);
   return hipGetLastError();
 }
 

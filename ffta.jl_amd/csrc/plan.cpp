@@ -86,7 +86,7 @@ static DevInfo& dev_info() {
 static thread_local bool g_dry = false;
 static thread_local int64_t g_dry_lds = 0;
 enum TokenKind { TK_STAGE = 1, TK_TWLO = 2, TK_TWHI = 3, TK_CHIRP = 4, TK_BHAT = 5, TK_IN = 6, TK_OUT = 7,
-                 TK_SCRATCH = 8, TK_BHATROWS = 9, TK_OFFTAB = 10, TK_TW16 = 11 };
+                 TK_SCRATCH = 8, TK_BHATROWS = 9, TK_OFFTAB = 10, TK_TW16 = 11, TK_CHIRPD = 12 };
 static void* token(int kind, int64_t v) {
   return (void*)(uintptr_t)((1ull << 60) | ((uint64_t)kind << 52) | (uint64_t)v);
 }
@@ -377,6 +377,7 @@ struct Tables {
   std::map<std::vector<int64_t>, std::unique_ptr<DevBuf>> stage;  // (L, radices) -> per-stage twiddles
   std::map<int64_t, std::pair<std::unique_ptr<DevBuf>, int>> two;  // M -> (lo|hi, sh)
   std::map<std::vector<int64_t>, std::unique_ptr<DevBuf>> tw16;     // (M, scale, amax) -> second factors
+  std::map<std::vector<int64_t>, std::unique_ptr<DevBuf>> chirp_d;  // (N, step, L) -> chirp steps
 };
 
 static size_t csize(int prec) { return prec == 32 ? 8 : 16; }
@@ -892,6 +893,33 @@ static fftahip_status attach_pass_twiddle(LinePass& p, const Launch& la, Tables&
   return tw16_table(T, M, p.tw_scale, amax, &p.tw16, s);
 }
 
+// FP64 chirp steps d[t] of a register pass with a chirp on load or store
+// (LinePass::chirp_d; cached per (N, step, L)).  Env FFTAHIP_CHIRP_CALC=0
+// keeps the table reads.
+static fftahip_status attach_chirp_steps(LinePass& p, const Launch& la, Tables& T, hipStream_t s) {
+  p.chirp_d = nullptr;
+  const bool cin = p.load_mode == LOAD_CHIRP, cout = p.store_mode == STORE_CHIRP;
+  if (!la.reg || (!cin && !cout) || env_i64("FFTAHIP_CHIRP_CALC", 1) == 0) return FFTAHIP_SUCCESS;
+  if (cin && cout && p.cae != p.cbe) return FFTAHIP_SUCCESS;
+  const int64_t step = cin ? p.cae : p.cbe;
+  if (step <= 0 || p.chirp_n <= 0) return FFTAHIP_SUCCESS;
+  if (g_dry) {
+    p.chirp_d = (const double2*)token(TK_CHIRPD, p.L);
+    return FFTAHIP_SUCCESS;
+  }
+  const std::vector<int64_t> key = {p.chirp_n, step, p.L};
+  auto it = T.chirp_d.find(key);
+  if (it == T.chirp_d.end()) {
+    auto buf = std::make_unique<DevBuf>();
+    fftahip_status st = buf->ensure((size_t)p.L * sizeof(double2));
+    if (st) return st;
+    HIP_TRY(launch_gen_chirp_steps((double2*)buf->p, step, p.chirp_n, p.L, s));
+    it = T.chirp_d.emplace(key, std::move(buf)).first;
+  }
+  p.chirp_d = (const double2*)it->second->p;
+  return FFTAHIP_SUCCESS;
+}
+
 static LinePass blank_pass() {
   LinePass p;
   std::memset(&p, 0, sizeof(p));
@@ -1044,6 +1072,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       if (st) return st;
       set_inner(p, d->inner[k == 1 ? 1 : 0], prec, la);
       if ((st = attach_pass_twiddle(p, la, T, M, stream))) return st;
+      if ((st = attach_chirp_steps(p, la, T, stream))) return st;
       la.lp = p;
       L.push_back(la);
     }
@@ -1137,6 +1166,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
     }
     set_inner(p, d->inner[s], prec, la);
     if ((st = attach_pass_twiddle(p, la, T, M, stream))) return st;
+    if ((st = attach_chirp_steps(p, la, T, stream))) return st;
     la.lp = p;
     L.push_back(la);
   }
@@ -2479,6 +2509,8 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
      << p.cb[2] << ", " << p.cb[3] << "], \"cm\": [" << p.cm[0] << ", " << p.cm[1] << ", " << p.cm[2] << ", "
      << p.cm[3] << "], \"cae\": " << p.cae << ", \"cbe\": " << p.cbe << ", \"cme\": " << p.cme << ", ";
   json_ptr(os, "chirp", p.chirp);
+  os << ", ";
+  json_ptr(os, "chirp_d", p.chirp_d);
   os << ", ";
   json_ptr(os, "bhat", p.bhat);
   os << "}";

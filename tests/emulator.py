@@ -14,8 +14,8 @@ from __future__ import annotations
 
 import numpy as np
 
-TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB, TK_TW16 = \
-    1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
+TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB, TK_TW16, \
+    TK_CHIRPD = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12
 EPT = 16
 STATIC_LDS = 6 * 8 * 256
 # register line passes: lines per workgroup, and the lengths with a codelet
@@ -176,6 +176,11 @@ class Emulator:
                 raise ScheduleError("register pass geometry")
             if p["mid_mode"] or not (p["col_in"] and p["col_out"]):
                 raise ScheduleError("register pass needs column layout on both sides and no mid product")
+            # computed chirp: the FP64 step table d[t] covers the line
+            if p["load_mode"] or p["store_mode"] == 2:
+                d = p.get("chirp_d")
+                if d is not None and (d[0] != TK_CHIRPD or d[1] < L):
+                    raise ScheduleError(f"chirp step table {d} for a line of {L}")
         else:
             if C < 1 or C > 256 or C * Tl > p["block"]:
                 raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
