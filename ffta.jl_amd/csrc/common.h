@@ -8,8 +8,14 @@ namespace fftahip {
 constexpr int kMaxStages = 12;      // radix stages inside one in-LDS line FFT
 constexpr int kMaxLines = 256;      // lines per workgroup, at most
 constexpr int kRegBlock = 256;      // lines (threads) per workgroup of a register line pass
-// static LDS of a line-pass workgroup: six int64 per line
-constexpr int kLineSharedBytes = 6 * 8 * kMaxLines;
+// per-line offset tables of a line-pass tile, kept in LDS after its lines:
+// six int64 per line
+constexpr int kLineTabBytes = 6 * 8;
+// LDS a tile of C lines of pitch ldl (elements of esz bytes) takes: the lines,
+// 16-byte aligned, then the per-line tables
+__host__ __device__ inline size_t tile_lds_bytes(int64_t C, int64_t ldl, int64_t esz) {
+  return (size_t)(((C * ldl * esz + 15) & ~(int64_t)15) + C * kLineTabBytes);
+}
 
 // Unsigned 32-bit division by a runtime constant: q = (umulhi(n, m) + n) >> s.
 // Exact for every n < 2^32 when d < 2^31 (round-up method of Granlund and

@@ -218,15 +218,13 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
 
 // ------------------------------------------------------------------ line pass
 
+// Per-line offsets of a tile (input, output, pass-twiddle index, chirp in /
+// out, spectrum), kept in LDS right after the tile's lines and sized by its
+// line count (tile_lds_bytes), so that a one-line tile does not reserve
+// tables for kMaxLines.
 struct LineShared {
-  int64_t in[kMaxLines];
-  int64_t out[kMaxLines];
-  int64_t twk[kMaxLines];
-  int64_t cin[kMaxLines];
-  int64_t cout[kMaxLines];
-  int64_t cmid[kMaxLines];
+  int64_t *in, *out, *twk, *cin, *cout, *cmid;
 };
-static_assert(sizeof(LineShared) == kLineSharedBytes, "static LDS size");
 
 // Kernel variants (compile-time), so that the common passes carry no
 // register cost for operations they do not use:
@@ -330,7 +328,7 @@ __device__ __forceinline__ double2 step_twiddle(const LinePass& p, int64_t twk) 
 
 template <class Rt, int EPT, int BLOCK, int MODE>
 __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& bp, uint32_t tile, cx<Rt>* lds,
-                                              LineShared& sh) {
+                                              const LineShared& sh) {
   constexpr int U = Batch<MODE>::U;
   const int tid = threadIdx.x;
   const int C = p.C;
@@ -741,7 +739,6 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __restrict__ tile_start, int ngroups,
                        BufPtrs bp, uint32_t ntiles) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ LineShared sh;
   const uint32_t bid = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   if (bid >= ntiles) return;
   int lo = 0, hi = ngroups - 1;
@@ -751,6 +748,9 @@ line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __re
     else hi = mid - 1;
   }
   const LinePass& p = passes[lo];
+  const int C = p.C;
+  int64_t* tab = reinterpret_cast<int64_t*>(smem_raw + (tile_lds_bytes(C, p.ldl, sizeof(cx<Rt>)) - (size_t)C * kLineTabBytes));
+  const LineShared sh{tab, tab + C, tab + 2 * C, tab + 3 * C, tab + 4 * C, tab + 5 * C};
   run_line_pass<Rt, EPT, BLOCK, MODE>(p, bp, bid - tile_start[lo], reinterpret_cast<cx<Rt>*>(smem_raw), sh);
 }
 
@@ -1233,7 +1233,7 @@ int pass_mode(const LinePass& p) {
 }
 
 void init_kernels(size_t lds_per_block) {
-  const size_t dyn = lds_per_block > (size_t)kLineSharedBytes ? lds_per_block - kLineSharedBytes : 0;
+  const size_t dyn = lds_per_block;
   allow_lds_modes<float, 256>(dyn);
   allow_lds_modes<float, 512>(dyn);
   allow_lds_modes<float, 1024>(dyn);

@@ -466,7 +466,9 @@ static int64_t lds_budget() {
   const int64_t cap = env_i64("FFTAHIP_LDS_BUDGET", 0);
   int64_t lds = g_dry ? g_dry_lds : dev_info().lds;
   if (lds < 65536) lds = 65536;
-  lds -= kLineSharedBytes + 1024;  // the kernel's static per-line tables
+  // headroom for the per-line tables of up to kMaxLines lines (a tile's
+  // LDS is its lines plus C * kLineTabBytes, tile_lds_bytes)
+  lds -= kLineTabBytes * kMaxLines + 1024;
   return cap > 0 ? std::min(cap, lds) : lds;
 }
 // Adjacent lines per workgroup for strided passes: 128-byte segments.
@@ -859,7 +861,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
     p.row_mid = wide ? 1 : 0;
     p.row_pad_ = 0;
   }
-  L.lds = (size_t)(C * ldl * esz);
+  L.lds = tile_lds_bytes(C, ldl, esz);
   L.grid = (p.nlines + C - 1) / C;
   p.xcd_remap = (L.grid >= 16 && L.grid % 8 == 0) ? 1 : 0;
   // short lines laid across memory on both sides (>= 128 contiguous bytes

@@ -17,7 +17,7 @@ import numpy as np
 TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB, TK_TW16, \
     TK_CHIRPD = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12
 EPT = 16
-STATIC_LDS = 6 * 8 * 256
+LINE_TAB_BYTES = 6 * 8  # per-line offset tables after a tile's lines
 # register line passes: lines per workgroup, and the lengths with a codelet
 REG_BLOCK = 256
 REG_LENGTHS = {n for n in range(2, 49) if all(q in (2, 3, 5, 7, 11, 13) for q in
@@ -186,10 +186,10 @@ class Emulator:
                 raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
             sh = p["lds_sh"]
             need = L + ((L - 1) >> sh if sh else 0)
-            if p["ldl"] < need or p["lds"] != C * p["ldl"] * self.esize:
+            if p["ldl"] < need or p["lds"] != ((C * p["ldl"] * self.esize + 15) & ~15) + C * LINE_TAB_BYTES:
                 raise ScheduleError("LDS size mismatch")
-            if p["lds"] + STATIC_LDS > lds_per_block:
-                raise ScheduleError(f"LDS {p['lds']} + static exceeds {lds_per_block}")
+            if p["lds"] > lds_per_block:
+                raise ScheduleError(f"LDS {p['lds']} exceeds {lds_per_block}")
             if p["grid"] * C < p["nlines"]:
                 raise ScheduleError("grid does not cover all lines")
             if p["xcd_remap"] and p["grid"] % 8:
