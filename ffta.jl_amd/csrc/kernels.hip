@@ -853,13 +853,24 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
   const Rt scale = (Rt)p.scale;
   const bool has_scale = p.scale != 1.0;
   double2 w = wr;
-  // computed output chirp (as on the load side): c[a + k*s] = c[a] w^k d[k]
-  const bool chirp_calc = store_mode == STORE_CHIRP && p.chirp_d;
-  double2 oc = make_double2(1.0, 0.0), ow = oc;
-  if (chirp_calc) {
+  // computed output chirp (as on the load side): c[a + k*s] = c[a] w^k d[k];
+  // such passes carry no post-twiddle (the planner attaches chirp steps to
+  // a chirp store only without one), so this loop holds no twiddle state
+  if (store_mode == STORE_CHIRP && p.chirp_d) {
     const uint64_t a = (uint64_t)o.cout, st = (uint64_t)p.cbe;
-    oc = chirp_root(a * a, p.chirp_n);
-    ow = chirp_root(2 * ((a * st) % (uint64_t)p.chirp_n), p.chirp_n);
+    double2 oc = chirp_root(a * a, p.chirp_n);
+    const double2 ow = chirp_root(2 * ((a * st) % (uint64_t)p.chirp_n), p.chirp_n);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const bool live = o.cout + (int64_t)k * p.cbe < p.chirp_n;
+      cx<Rt> y = cconj(cmul(from_d2<Rt>(dmul2(oc, uld(p.chirp_d + k))), v[k]));
+      oc = dmul2(oc, ow);
+      y = cconj_if(y, sgn_out);
+      if (has_scale) y = cscale(y, scale);
+      if (live) gst(out + (int64_t)k * p.be, y);
+      if (k % kRegChunk == kRegChunk - 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    return;
   }
 #pragma unroll
   for (int k = 0; k < L; ++k) {
@@ -878,12 +889,7 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
     } else if (store_mode == STORE_CHIRP) {
       const int64_t n = o.cout + (int64_t)k * p.cbe;
       live = n < p.chirp_n;
-      if (chirp_calc) {
-        y = cconj(cmul(from_d2<Rt>(dmul2(oc, uld(p.chirp_d + k))), y));
-        oc = dmul2(oc, ow);
-      } else {
-        y = cconj(cmul(gld(chirp + (live ? n : 0)), y));
-      }
+      y = cconj(cmul(gld(chirp + (live ? n : 0)), y));
     }
     y = cconj_if(y, sgn_out);
     if (has_scale) y = cscale(y, scale);

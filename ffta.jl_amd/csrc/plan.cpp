@@ -903,6 +903,7 @@ static fftahip_status attach_chirp_steps(LinePass& p, const Launch& la, Tables& 
   const bool cin = p.load_mode == LOAD_CHIRP, cout = p.store_mode == STORE_CHIRP;
   if (!la.reg || (!cin && !cout) || env_i64("FFTAHIP_CHIRP_CALC", 1) == 0) return FFTAHIP_SUCCESS;
   if (cin && cout && p.cae != p.cbe) return FFTAHIP_SUCCESS;
+  if (cout && p.pretw == 2) return FFTAHIP_SUCCESS;  // the computed store path carries no post-twiddle
   const int64_t step = cin ? p.cae : p.cbe;
   if (step <= 0 || p.chirp_n <= 0) return FFTAHIP_SUCCESS;
   if (g_dry) {
