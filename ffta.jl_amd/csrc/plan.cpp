@@ -1341,11 +1341,30 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
     // 2N-1 gives the same convolution)
     int64_t chosen = best_line_length(N, prec);
     int64_t first = 0;
-    for (int64_t M = 2 * N - 1; M <= 4 * (2 * N - 1) && !chosen; ++M) {
+    // three fused passes: among the splittable 5-smooth M up to 15% above
+    // the first, the least M * (row stages + 3) (the column passes and the
+    // row pass's own load, spectrum and store phases cost about three
+    // stages per element); env FFTAHIP_BLUE_COST=0 takes the first
+    const bool costed = env_i64("FFTAHIP_BLUE_COST", 1) != 0;
+    double best_cost = 1e300;
+    int64_t limit = chosen ? 0 : 4 * (2 * N - 1);  // one kernel: no three-pass search
+    for (int64_t M = 2 * N - 1; M <= limit; ++M) {
       if (!smooth5(M)) continue;
       if (!first) first = M;
       int64_t a, b;
-      if (blue3_split(M, prec, &a, &b)) chosen = M;
+      if (!blue3_split(M, prec, &a, &b)) continue;
+      if (!chosen) limit = std::min(limit, M + M * 15 / 100);
+      if (!costed) {
+        chosen = M;
+        break;
+      }
+      Inner row;
+      make_inner(b, row, prec);
+      const double cost = (double)M * ((double)row.rad.size() + 3.0);
+      if (cost < best_cost) {
+        best_cost = cost;
+        chosen = M;
+      }
     }
     lp.M = chosen ? chosen : first;
   } else if (!lp.blue) {
