@@ -91,7 +91,10 @@ struct LinePass {
   // 1: the load / spectrum / store phase of a row-layout side runs stepped,
   // one line per block/C threads (each access of a wave covers >= 128
   // contiguous bytes of the line, or the whole line)
-  int32_t row_in, row_mid, row_out, row_pad_;
+  int32_t row_in, row_mid, row_out;
+  // 1: the spectrum product is applied by the second sweep's first radix
+  // stage as it reads its inputs (no separate spectrum phase)
+  int32_t mid_in_stage;
   FastDiv divC, divL;       // by C and by L
   FastDiv divR;             // by block / C
   int64_t nlines;           // < 2^31

@@ -110,7 +110,7 @@ __device__ __forceinline__ int pad_idx(int i, int she) { return i + (i >> she); 
 template <class Rt, int RAD, int EPT, bool FUSE, class LdF, class StF>
 __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl, int L, int Ns, FastDiv nsd,
                                       const cx<Rt>* __restrict__ tw, int sh, bool gload, bool gstore, const LdF& ld,
-                                      const StF& st) {
+                                      const StF& st, const cx<Rt>* __restrict__ pm, int64_t pms) {
   constexpr int NB = EPT / RAD;
   const int Lr = L / RAD;
   const int she = pad_shift(sh);
@@ -123,7 +123,25 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = jl + b * Tl;
-    if (active && j < Lr) {
+    if (active && j < Lr && pm) {
+      // a spectrum product applied on the way in (first stage of a sweep:
+      // no twiddles): v = conj(v * pm[i * pms]), its factors requested
+      // before the LDS reads
+      cx<Rt> f[RAD];
+#pragma unroll
+      for (int t = 0; t < RAD; ++t) f[t] = gld(pm + (int64_t)(j + t * Lr) * pms);
+      if (rstep) {
+        const int p0 = pad_idx(j, she);
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) v[b][t] = line[p0 + t * rstride];
+      } else {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, she)];
+      }
+#pragma unroll
+      for (int t = 0; t < RAD; ++t) v[b][t] = cconj(cmul(v[b][t], f[t]));
+      dft_any<Rt, RAD>(v[b]);
+    } else if (active && j < Lr) {
       // FP32: the stage twiddles are requested first (global loads, L2
       // latency) and consumed in the same branch region, after the LDS
       // reads.  FP64 loads them as it consumes them (a full set in flight
@@ -187,7 +205,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
 template <class Rt, int EPT, bool FUSE, class LdF, class StF>
 __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const LinePass& p,
                                       const cx<Rt>* __restrict__ tw, bool fuse_first, bool fuse_last, const LdF& ld,
-                                      const StF& st) {
+                                      const StF& st, const cx<Rt>* __restrict__ pm = nullptr, int64_t pms = 0) {
   int Ns = 1;
   const int sh = p.lds_sh;
   const int nst = p.nst;
@@ -198,7 +216,7 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
     const bool gs = fuse_last && s == nst - 1;
     switch (r) {
 #define FFTAHIP_STAGE(R) \
-  case R: stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st); break;
+  case R: stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == 0 ? pm : nullptr, pms); break;
       FFTAHIP_STAGE(2)
       FFTAHIP_STAGE(3)
       FFTAHIP_STAGE(4)
@@ -530,7 +548,9 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   constexpr bool kFuse = MODE == MODE_PLAIN;
   sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, fuse_in, fuse_out && !has_mid, ld, st);
 
-  if (has_mid) {
+  if (has_mid && p.mid_in_stage) {
+    sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, false, fuse_out, ld, st, bhat + sh.cmid[cs], p.cme);
+  } else if (has_mid) {
     if (p.row_mid && col_step) {
       if (row_c < nvalid) {
         const int c = row_c;

@@ -859,7 +859,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
     p.row_in = (wide && !p.col_in && !p.fuse_in) ? 1 : 0;
     p.row_out = (wide && !p.col_out && !p.fuse_out) ? 1 : 0;
     p.row_mid = wide ? 1 : 0;
-    p.row_pad_ = 0;
+    p.mid_in_stage = (p.mid_mode == MID_BHAT && env_i64("FFTAHIP_MID_STAGE", 1) != 0) ? 1 : 0;
   }
   L.lds = tile_lds_bytes(C, ldl, esz);
   L.grid = (p.nlines + C - 1) / C;
