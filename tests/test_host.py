@@ -240,3 +240,45 @@ def test_autotune_table(tmp_path):
         assert autotune.schedule(32, 4096) == [8, 8, 8, 8]
     finally:
         autotune.clear()
+
+
+def _chirp_direct(n: int, N: int) -> complex:
+    # the chirp table's definition: exp(i pi r / N), r = n^2 mod 2N folded into (-N, N]
+    r = (n * n) % (2 * N)
+    if r > N:
+        r -= 2 * N
+    return complex(np.cos(np.pi * r / N), np.sin(np.pi * r / N))
+
+
+def _chirp_root(m: int, N: int) -> complex:
+    # kernels.hip chirp_root: (m mod 2N) folded, with m < 2^64
+    assert 0 <= m < 1 << 64
+    return _chirp_direct_r(m % (2 * N), N)
+
+
+def _chirp_direct_r(r: int, N: int) -> complex:
+    if r > N:
+        r -= 2 * N
+    return complex(np.cos(np.pi * r / N), np.sin(np.pi * r / N))
+
+
+@pytest.mark.parametrize("N", [73, 1009, 20011, 1000003, (1 << 31) - 1])
+def test_register_pass_chirp_factorisation(N):
+    """The register passes form the chirp of positions n = a + t*s as
+    c[a] * w^t * d[t] with the kernel's unsigned 64-bit reductions: a*a and
+    2*((a*s) mod N) for c[a] and w, ((t*s) mod 2N)^2 for the step table d[t].
+    Each intermediate stays below 2^64 for N < 2^31, and the product equals
+    the table's chirp at every position (incl. positions a + t*s >= 2N)."""
+    rng = np.random.default_rng(N)
+    for _ in range(40):
+        a = int(rng.integers(0, N))
+        s = int(rng.integers(1, N))
+        L = int(rng.integers(2, 49))
+        c = _chirp_root(a * a, N)
+        w = _chirp_root(2 * ((a * s) % N), N)
+        for t in range(L):
+            u = (t * s) % (2 * N)
+            d = _chirp_root(u * u, N)
+            got = c * (w ** t) * d
+            want = _chirp_direct(a + t * s, N)
+            assert abs(got - want) < 1e-9, (N, a, s, t)
