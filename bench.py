@@ -182,10 +182,21 @@ def cpu_baseline(lengths, dirs, seconds_target: float = 15.0, seed: int = 1):
             break
         grow = max(2.0, min(8.0, seconds_target / max(dt, 1e-3)))
         take = min(len(lengths), int(take * grow))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": round(flops_of(L) / dt / 1e9, 3),
         "unit": "GFLOP/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "cpu_model": model,
         "kind": "port",
         "sample": f"{take} of {len(lengths)} transforms (random subset, seed {seed}), ComplexF32, "
                   f"{dt:.1f} s wall on {threads} threads, one transform per thread",
