@@ -125,11 +125,15 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
     const int j = jl + b * Tl;
     if (active && j < Lr && pm) {
       // a spectrum product applied on the way in (first stage of a sweep:
-      // no twiddles): v = conj(v * pm[i * pms]), its factors requested
-      // before the LDS reads
-      cx<Rt> f[RAD];
+      // no twiddles): v = conj(v * pm[i * pms]).  FP32 requests its factors
+      // before the LDS reads; FP64 loads each as it consumes it (register
+      // budget)
+      constexpr bool kPre = sizeof(Rt) == 4;
+      cx<Rt> f[kPre ? RAD : 1];
+      if (kPre) {
 #pragma unroll
-      for (int t = 0; t < RAD; ++t) f[t] = gld(pm + (int64_t)(j + t * Lr) * pms);
+        for (int t = 0; t < RAD; ++t) f[kPre ? t : 0] = gld(pm + (int64_t)(j + t * Lr) * pms);
+      }
       if (rstep) {
         const int p0 = pad_idx(j, she);
 #pragma unroll
@@ -139,7 +143,8 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
         for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, she)];
       }
 #pragma unroll
-      for (int t = 0; t < RAD; ++t) v[b][t] = cconj(cmul(v[b][t], f[t]));
+      for (int t = 0; t < RAD; ++t)
+        v[b][t] = cconj(cmul(v[b][t], kPre ? f[kPre ? t : 0] : gld(pm + (int64_t)(j + t * Lr) * pms)));
       dft_any<Rt, RAD>(v[b]);
     } else if (active && j < Lr) {
       // FP32: the stage twiddles are requested first (global loads, L2
