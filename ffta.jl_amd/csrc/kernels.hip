@@ -147,11 +147,13 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
         v[b][t] = cconj(cmul(v[b][t], kPre ? f[kPre ? t : 0] : gld(pm + (int64_t)(j + t * Lr) * pms)));
       dft_any<Rt, RAD>(v[b]);
     } else if (active && j < Lr) {
-      // FP32: the stage twiddles are requested first (global loads, L2
-      // latency) and consumed in the same branch region, after the LDS
-      // reads.  FP64 loads them as it consumes them (a full set in flight
-      // would not fit the register budget).
-      constexpr bool kPre = sizeof(Rt) == 4;
+      // FP32 pre-twiddle and full passes: the stage twiddles are requested
+      // first (global loads, L2 latency) and consumed in the same branch
+      // region, after the LDS reads.  FP64 loads them as it consumes them (a
+      // full set in flight would not fit the register budget), and so does
+      // the FP32 plain variant, which is built for 7 resident waves per SIMD
+      // (<= 72 VGPRs) and hides the latency with waves instead.
+      constexpr bool kPre = sizeof(Rt) == 4 && !FUSE;
       const cx<Rt>* __restrict__ w = tw;
       cx<Rt> wv[RAD > 1 ? RAD - 1 : 1];
       if (Ns > 1) {
@@ -757,10 +759,11 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
 // operations.  Tiles are visited in XCD-aware order (workgroups are dealt
 // round-robin to the 8 XCDs, so tile = (bid % 8) * (grid / 8) + bid / 8 with
 // grid a multiple of 8 gives each XCD a contiguous range of tiles and its
-// L2 the neighbouring lines).  At least 4 waves per SIMD (<= 128 VGPRs): the
-// passes are latency bound on global memory.
+// L2 the neighbouring lines).  At least 4 waves per SIMD (<= 128 VGPRs), 7
+// for the FP32 plain variant (<= 72 VGPRs): the passes are latency bound on
+// global memory.
 template <class Rt, int EPT, int BLOCK, int MODE>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MODE == MODE_PLAIN && sizeof(Rt) == 4 ? 7 : 4)))
 line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __restrict__ tile_start, int ngroups,
                        BufPtrs bp, uint32_t ntiles) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
