@@ -161,3 +161,19 @@ def test_set_scale_changes_result(torch_cuda):
     mul_(dy, p, dx)
     y2 = to_host(torch_cuda, ty, x.shape)
     assert oracle.relerr(y2, 0.25 * y1.astype(np.complex128)) <= 1e-7
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("n", [20011, 62539])
+@pytest.mark.parametrize("setting", ["FFTAHIP_CHIRP_CALC=0", "FFTAHIP_BLUE_COST=0", "FFTAHIP_ROWSTEP=0"])
+def test_planner_settings_vs_oracle(torch_cuda, monkeypatch, n, precision, setting):
+    """The planner's alternative paths kept behind environment settings (chirp
+    table reads in the register passes, the first splittable Bluestein
+    length, the flat element mapping of row phases) still match the oracle."""
+    name, value = setting.split("=")
+    monkeypatch.setenv(name, value)
+    dtype = np.complex64 if precision == 32 else np.complex128
+    for direction in (-1, 1):
+        x = _rand(n, dtype, seed=n + direction)
+        y = _gpu_transform(torch_cuda, x, direction)
+        assert oracle.relerr(y, _oracle(x, direction)) <= tol(n, precision)
