@@ -7,6 +7,39 @@
 namespace fftahip {
 
 void init_kernels(size_t lds_per_block);
+
+// A line-pass kernel variant: its host entry (for hipLaunchKernel and graph
+// kernel nodes) and workgroup size.
+struct KernelRef {
+  const void* fn;
+  int block;
+};
+KernelRef lds_kernel(int precision, int block, int mode);
+KernelRef reg_kernel(int precision, int mode);
+// Kernel arguments of a work-list launch (passes, tile starts, group count,
+// buffers, tile count), laid out for hipLaunchKernel / kernel nodes.
+struct LaunchArgs {
+  const LinePass* passes;
+  const uint32_t* tiles;
+  int ngroups;
+  BufPtrs bp;
+  uint32_t ntiles;
+  void* args[5];
+  LaunchArgs(const LinePass* P, const uint32_t* T, int ng, const BufPtrs& b, int64_t grid);
+  LaunchArgs(const LaunchArgs&) = delete;
+  LaunchArgs& operator=(const LaunchArgs&) = delete;
+};
+// Workgroups a launch of `grid` tiles takes (a multiple of 8).
+uint32_t padded_grid(int64_t grid);
+// Per-variant entries (kern_lds.hip / kern_reg.hip translation units).
+const void* lds_fn_f32_256(int mode);
+const void* lds_fn_f32_512(int mode);
+const void* lds_fn_f32_1024(int mode);
+const void* lds_fn_f64_256(int mode);
+const void* lds_fn_f64_512(int mode);
+const void* lds_fn_f64_1024(int mode);
+const void* reg_fn_f32(int mode);
+const void* reg_fn_f64(int mode);
 // Kernel variant a pass needs (0 plain, 1 load-side pass twiddle, 2 full).
 int pass_mode(const LinePass& p);
 // Launch ngroups independent passes (device descriptors) as one grid.
