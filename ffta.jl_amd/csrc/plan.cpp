@@ -655,8 +655,14 @@ struct Launch {
   size_t bytes = 0;
 };
 
+// Line indices, per-slot extents and in-line positions are 32-bit on the
+// device (FastDiv needs divisors below 2^31): a line set must stay below this.
+constexpr int64_t kMaxLinesPerPass = (int64_t(1) << 31) - 1;
+
 // Fill geometry: dims (<= 3 affine + optional table dim) -> LinePass fields.
-static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
+// False (nothing usable written) when the line count or an extent reaches
+// 2^31.
+static bool set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
   // unit dims carry nothing; the table dim (if any) stays last
   std::vector<Dim> D;
   for (size_t i = 0; i < Din.size(); ++i)
@@ -674,7 +680,11 @@ static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
   for (size_t i = 0; i < naff; ++i) slots[i] = D[i];
   if (tab) slots[3] = D.back();
   int64_t total = 1;
-  for (int i = 0; i < 4; ++i) total *= slots[i].n;
+  for (int i = 0; i < 4; ++i) {
+    if (slots[i].n < 1 || slots[i].n > kMaxLinesPerPass) return false;
+    total *= slots[i].n;
+    if (total > kMaxLinesPerPass) return false;
+  }
   p.nlines = total;
   for (int i = 0; i < 3; ++i) {
     p.n[i] = slots[i].n;
@@ -691,6 +701,11 @@ static void set_dims(LinePass& p, std::vector<Dim> Din, bool tab) {
   }
   p.col_in = (p.ae != 1 && slots[0].n > 1 && slots[0].a == 1) ? 1 : 0;
   p.col_out = (p.be != 1 && slots[0].n > 1 && slots[0].b == 1) ? 1 : 0;
+  return true;
+}
+
+static fftahip_status lines_capacity_error() {
+  return fail(FFTAHIP_ERR_CAPACITY, "a pass over this array needs 2^31 or more lines (line indices are 32-bit)");
 }
 
 // ------------------------------------------------------------------ LDS bank model
@@ -760,7 +775,12 @@ static bool reg_eligible(const LinePass& p, const Inner& in, int prec) {
   return p.n[0] * (int64_t)csize(prec) >= 128;
 }
 
-static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
+// allow_reg = false keeps the pass in LDS even when its lines qualify for the
+// register path: a transform whose lines fit one in-LDS pass then computes
+// the same arithmetic (radix schedule, stage twiddles) whether its lines are
+// contiguous or strided, so a 1-D plan along any dimension equals the 1-D
+// transform of each slice bit for bit (test/onedim/complex_forward.jl:27-33).
+static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool allow_reg = true) {
   p.L = (int32_t)in.L;
   p.nst = (int32_t)in.rad.size();
   int64_t Ns = 1;
@@ -866,7 +886,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L) {
   p.xcd_remap = (L.grid >= 16 && L.grid % 8 == 0) ? 1 : 0;
   // short lines laid across memory on both sides (>= 128 contiguous bytes
   // per wave access): one line per thread in registers
-  if (reg_eligible(p, in, prec)) {
+  if (allow_reg && reg_eligible(p, in, prec)) {
     L.reg = true;
     L.block = kRegBlock;
     L.lds = 0;
@@ -976,7 +996,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
         if (ls.n[i] > 1) D.push_back(Dim{ls.n[i], ls.in_str[i], ls.out_str[i], 0, 0, false});
       if (D.empty()) D.push_back(Dim{1, 0, 0, 0, 0, false});
     }
-    set_dims(p, D, tab);
+    if (!set_dims(p, D, tab)) return lines_capacity_error();
     st = stage_table(T, d->inner[0], &p.tw, stream);
     if (st) return st;
     p.conj_in = conj ? 1 : 0;
@@ -992,7 +1012,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       p.chirp = lp.chirp_ptr;
       p.bhat = lp.bhat_ptr;
     }
-    set_inner(p, d->inner[0], prec, la);
+    set_inner(p, d->inner[0], prec, la, false);
     la.lp = p;
     L.push_back(la);
     return FFTAHIP_SUCCESS;
@@ -1068,7 +1088,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
         p.cbe = M2;
         p.conj_out = conj ? 1 : 0;
       }
-      set_dims(p, D, tab);
+      if (!set_dims(p, D, tab)) return lines_capacity_error();
       if (k == 0) p.in_tab = tab ? ls.in_tab : nullptr;
       if (k == 2) p.out_tab = tab ? ls.out_tab : nullptr;
       st = stage_table(T, d->inner[k == 1 ? 1 : 0], &p.tw, stream);
@@ -1131,7 +1151,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
         sstride *= ls.n[b];
       }
     }
-    set_dims(p, D, tab);
+    if (!set_dims(p, D, tab)) return lines_capacity_error();
     st = stage_table(T, d->inner[s], &p.tw, stream);
     if (st) return st;
     if (Ns > 1) {
@@ -1458,6 +1478,9 @@ struct fftahip_batch_s {
   }
 };
 
+static fftahip_status plan_shape_schedule(fftahip_plan plan, int ndims, const int64_t* dims, DevSchedule** out,
+                                          hipStream_t s);
+
 extern "C" {
 
 const char* fftahip_version(void) { return "fftahip 0.1.0 (gfx950)"; }
@@ -1505,6 +1528,16 @@ fftahip_status fftahip_plan_create(fftahip_plan* plan, int precision, int direct
   hipStream_t s = nullptr;
   for (auto& lp : p->lens) {
     fftahip_status st = prepare_length(*lp, p->tables, s);
+    if (st) return st;
+  }
+  // the schedule of the creation shape is built and uploaded now (with a
+  // device present), so that the first execution and a stream capture find
+  // it ready
+  int64_t total = 1;
+  for (int i = 0; i < ndims; ++i) total *= dims[i];
+  if (!g_dry && total > 0 && dev_info().ok) {
+    DevSchedule* S;
+    fftahip_status st = plan_shape_schedule(p.get(), ndims, dims, &S, s);
     if (st) return st;
   }
   *plan = p.release();
@@ -1715,9 +1748,21 @@ static fftahip_status plan_shape_schedule(fftahip_plan plan, int ndims, const in
   return FFTAHIP_SUCCESS;
 }
 
+// The argument checks every execution applies before it launches anything:
+// the shape must fit the plan as mul!(y, p, x) requires (x and y share dims
+// here), with the plan's own region (src/plan.jl:157-164, 170-182,
+// 210-222, 260-272).
+static fftahip_status check_shape(fftahip_plan plan, int ndims, const int64_t* dims) {
+  if (ndims < 0 || (ndims > 0 && !dims)) return fail(FFTAHIP_ERR_ARGUMENT, "dims is NULL");
+  for (int i = 0; i < ndims; ++i)
+    if (dims[i] < 0) return fail(FFTAHIP_ERR_ARGUMENT, "negative dimension");
+  return fftahip_plan_check(plan, (int)plan->region.size(), plan->region.data(), ndims, dims, ndims, dims);
+}
+
 fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
                                void* stream) {
   if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  if (fftahip_status cs = check_shape(plan, ndims, dims)) return cs;
   hipStream_t s = (hipStream_t)stream;
   int64_t total = 1;
   for (int i = 0; i < ndims; ++i) total *= dims[i];
@@ -1740,6 +1785,7 @@ fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims
 fftahip_status fftahip_execute_host(fftahip_plan plan, int ndims, const int64_t* dims, const void* h_in,
                                     void* h_out) {
   if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  if (fftahip_status cs = check_shape(plan, ndims, dims)) return cs;
   int64_t total = 1;
   for (int i = 0; i < ndims; ++i) total *= dims[i];
   const size_t bytes = (size_t)total * csize(plan->prec);
@@ -1864,11 +1910,8 @@ fftahip_status fftahip_rplan_create(fftahip_rplan* plan, int precision, int dire
   int64_t P = 1, Q = 1;
   for (int i = 0; i < d1; ++i) P *= D[i];
   for (int i = d1 + 1; i < nd; ++i) Q *= D[i];
-  int64_t totalX = 1, totalY = 1;
-  for (int i = 0; i < nd; ++i) {
-    totalX *= X[i];
-    totalY *= Y[i];
-  }
+  int64_t totalX = 1;
+  for (int i = 0; i < nd; ++i) totalX *= X[i];
   if (P * std::max(n, h) * Q >= (int64_t(1) << 31))
     return fail(FFTAHIP_ERR_CAPACITY, "real transform needs more than 2^31 - 1 elements");
   const bool even = n % 2 == 0;
@@ -2749,6 +2792,7 @@ fftahip_status fftahip_debug_fastdiv(uint32_t d, const uint32_t* n, uint32_t* q,
 // plan_internal.h; C++ linkage, outside the extern "C" blocks).
 namespace fftahip {
 fftahip_status plan_scratch_bytes(fftahip_plan plan, int ndims, const int64_t* dims, size_t* bytes) {
+  if (fftahip_status cs = check_shape(plan, ndims, dims)) return cs;
   DevSchedule* S;
   fftahip_status st = plan_shape_schedule(plan, ndims, dims, &S, nullptr);
   if (st) return st;
@@ -2757,6 +2801,7 @@ fftahip_status plan_scratch_bytes(fftahip_plan plan, int ndims, const int64_t* d
 }
 fftahip_status plan_execute_scratch(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
                                     void* scratch, hipStream_t s) {
+  if (fftahip_status cs = check_shape(plan, ndims, dims)) return cs;
   DevSchedule* S;
   fftahip_status st = plan_shape_schedule(plan, ndims, dims, &S, s);
   if (st) return st;

@@ -147,23 +147,27 @@ PlanKey key_of(const Request& r) {
   return k;
 }
 
-// Plan and scratch size for a request's shape (created on first use).
-fftahip_status get_plan(fftahip_server_s* S, const Request& r, fftahip_plan* p, size_t* scratch) {
+// Plan and scratch size for a request's shape (created on first use).  Only
+// the dispatcher thread inserts plans, so a new plan is built with the lock
+// released (`lk`, held on entry and on return): plan creation runs table
+// kernels and synchronises (milliseconds for a new Bluestein length), and
+// submitters, waiters and stats readers must not stall behind it.
+fftahip_status get_plan(fftahip_server_s* S, const Request& r, fftahip_plan* p, size_t* scratch,
+                        std::unique_lock<std::mutex>& lk) {
   PlanKey k = key_of(r);
   auto it = S->plans.find(k);
   if (it == S->plans.end()) {
     std::vector<int64_t> region(r.ndims);
     for (int i = 0; i < r.ndims; ++i) region[i] = i + 1;
     fftahip_plan plan = nullptr;
+    size_t sb = 0;
+    lk.unlock();
     fftahip_status st = fftahip_plan_create(&plan, r.prec, r.dir, r.ndims, r.dims, r.ndims, region.data(),
                                             FFTAHIP_DEFAULT_BLUESTEIN_CUTOFF);
+    if (!st) st = fftahip::plan_scratch_bytes(plan, r.ndims, r.dims, &sb);
+    if (st && plan) fftahip_plan_destroy(plan);
+    lk.lock();
     if (st) return st;
-    size_t sb = 0;
-    st = fftahip::plan_scratch_bytes(plan, r.ndims, r.dims, &sb);
-    if (st) {
-      fftahip_plan_destroy(plan);
-      return st;
-    }
     it = S->plans.emplace(k, plan).first;
     S->scratch_bytes[k] = sb;
   }
@@ -242,7 +246,7 @@ void worker_loop(fftahip_server_s* S) {
       Request* r = S->pending.front();
       fftahip_plan plan = nullptr;
       size_t scratch = 0;
-      fftahip_status st = get_plan(S, *r, &plan, &scratch);
+      fftahip_status st = get_plan(S, *r, &plan, &scratch, lk);
       if (st) {
         S->pending.pop_front();
         finish(S, r, st, now_s());
@@ -369,6 +373,12 @@ fftahip_status fftahip_server_replay(fftahip_server s, int64_t count, const doub
                                      const int* direction, const int* ndims, const int64_t* dims, void* const* in,
                                      void* const* out, int host_buffers, uint64_t* tickets) {
   if (!s) return set_error(FFTAHIP_ERR_ARGUMENT, "NULL server");
+  if (count < 0 || (count > 0 && (!t_offset || !precision || !direction || !ndims || !dims || !in || !out ||
+                                   !tickets)))
+    return set_error(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  // dims holds 3 entries per request
+  for (int64_t i = 0; i < count; ++i)
+    if (ndims[i] < 1 || ndims[i] > 3) return set_error(FFTAHIP_ERR_ARGUMENT, "replay requests have 1 to 3 dimensions");
   const double t0 = now_s();
   for (int64_t i = 0; i < count; ++i) {
     const double due = t0 + t_offset[i];

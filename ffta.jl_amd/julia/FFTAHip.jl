@@ -48,9 +48,15 @@ mutable struct FFTAHipPlan{T<:Union{ComplexF32,ComplexF64},N} <: AbstractFFTs.Pl
     dir::Cint
 end
 
+# A region as FFTA accepts it (an Int, a tuple, a vector or a range), sorted
+# as _sort does (src/plan.jl:51-69).  collect(2) is a 0-dimensional array,
+# so an Int is wrapped rather than collected.
+region_vector(region::Integer) = Int64[region]
+region_vector(region) = sort!(Int64[r for r in region])
+
 function FFTAHipPlan{T,N}(x::AbstractArray{T}, region, dir::Cint, cutoff::Int) where {T,N}
     dims = Int64[size(x)...]
-    reg = Int64[sort(collect(region))...]
+    reg = region_vector(region)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:fftahip_plan_create, libfftahip), Cint,
                 (Ptr{Ptr{Cvoid}}, Cint, Cint, Cint, Ptr{Int64}, Cint, Ptr{Int64}, Int64),
@@ -66,7 +72,7 @@ end
 
 # src/plan.jl:71-100
 _plan(x::AbstractArray{T}, region, dir; BLUESTEIN_CUTOFF=DEFAULT_BLUESTEIN_CUTOFF, _kwargs...) where {T} =
-    FFTAHipPlan{T,length(region)}(x, region, dir, BLUESTEIN_CUTOFF)
+    FFTAHipPlan{T,length(region_vector(region))}(x, region, dir, BLUESTEIN_CUTOFF)
 AbstractFFTs.plan_fft(x::AbstractArray{T}, region; kwargs...) where {T<:Union{ComplexF32,ComplexF64}} =
     _plan(x, region, FFT_FORWARD; kwargs...)
 AbstractFFTs.plan_bfft(x::AbstractArray{T}, region; kwargs...) where {T<:Union{ComplexF32,ComplexF64}} =
@@ -90,20 +96,60 @@ function Base.size(p::FFTAHipPlan{T,N}, i::Int) where {T,N}
 end
 Base.size(p::FFTAHipPlan{T,N}) where {T,N} = ntuple(i -> size(p, i), Val(N))
 
-# src/plan.jl:157-285: argument checks in the library, then the transform
-function LinearAlgebra.mul!(y::StridedArray{T}, p::FFTAHipPlan{T}, x::StridedArray{T}) where {T}
+"""
+    device_pointer(x) -> Union{Ptr{Cvoid},Nothing}
+
+Device address of the data of `x` when `x` is a dense array in GPU memory,
+else `nothing` (the default).  A GPU array package adds a method for its
+array type, e.g. for AMDGPU.jl
+
+    FFTAHip.device_pointer(x::ROCArray) = Ptr{Cvoid}(UInt(pointer(x)))
+    FFTAHip.stream_handle(x::ROCArray) = AMDGPU.stream().stream
+
+and `mul!`/`*` then run `fftahip_execute` on the device arrays directly
+(asynchronous on that stream, no host staging).
+"""
+device_pointer(x) = nothing
+"""hipStream_t (as `Ptr{Cvoid}`) the transform of device array `x` is issued on; C_NULL = default stream."""
+stream_handle(x) = C_NULL
+
+# Dense column-major host data (what the library reads through a plain pointer).
+is_dense(x::DenseArray) = true
+is_dense(x::StridedArray) = strides(x) == Base.size_to_strides(1, size(x)...)
+is_dense(x) = false
+
+# src/plan.jl:157-285: argument checks in the library, then the transform.
+# Any AbstractArray is accepted, as FFTA's mul! methods do: device arrays
+# run in place on the device, dense host arrays are staged through device
+# memory by fftahip_execute_host, other host arrays (non-unit-stride views,
+# lazy arrays) through a dense copy.
+function LinearAlgebra.mul!(y::AbstractArray{T}, p::FFTAHipPlan{T}, x::AbstractArray{T}) where {T}
     Base.require_one_based_indexing(x, y)
-    reg = Int64[(p.region isa Int ? [p.region] : p.region)...]
+    reg = region_vector(p.region)
     dx = Int64[size(x)...]
     dy = Int64[size(y)...]
     check(ccall((:fftahip_plan_check, libfftahip), Cint,
                 (Ptr{Cvoid}, Cint, Ptr{Int64}, Cint, Ptr{Int64}, Cint, Ptr{Int64}),
                 p.handle, length(reg), reg, length(dx), dx, length(dy), dy))
-    GC.@preserve x y begin
+    px, py = device_pointer(x), device_pointer(y)
+    if px !== nothing && py !== nothing
+        GC.@preserve x y begin
+            check(ccall((:fftahip_execute, libfftahip), Cint,
+                        (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                        p.handle, length(dx), dx, px, py, stream_handle(x)))
+        end
+        return y
+    end
+    (px === nothing && py === nothing) ||
+        throw(ArgumentError("x and y must both be device arrays or both host arrays"))
+    xs = is_dense(x) ? x : Array(x)
+    ys = is_dense(y) ? y : Array{T}(undef, size(y))
+    GC.@preserve xs ys begin
         check(ccall((:fftahip_execute_host, libfftahip), Cint,
                     (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}),
-                    p.handle, length(dx), dx, pointer(x), pointer(y)))
+                    p.handle, length(dx), dx, pointer(xs), pointer(ys)))
     end
+    ys === y || copyto!(y, ys)
     return y
 end
 
@@ -150,17 +196,17 @@ end
 
 AbstractFFTs.plan_rfft(x::AbstractArray{T,N}, region; BLUESTEIN_CUTOFF=DEFAULT_BLUESTEIN_CUTOFF,
                        _kwargs...) where {T<:Union{Float32,Float64},N} =
-    _rplan(T, size(x), sort(collect(region)), FFT_FORWARD, 0, BLUESTEIN_CUTOFF)
+    _rplan(T, size(x), region_vector(region), FFT_FORWARD, 0, BLUESTEIN_CUTOFF)
 
 AbstractFFTs.plan_brfft(x::AbstractArray{T,N}, len::Int, region; BLUESTEIN_CUTOFF=DEFAULT_BLUESTEIN_CUTOFF,
                         _kwargs...) where {T<:Union{ComplexF32,ComplexF64},N} =
-    _rplan(T, size(x), sort(collect(region)), FFT_BACKWARD, len, BLUESTEIN_CUTOFF)
+    _rplan(T, size(x), region_vector(region), FFT_BACKWARD, len, BLUESTEIN_CUTOFF)
 
 function Base.:*(p::FFTAHipRPlan{T,N}, x::AbstractArray{T,N}) where {T,N}
     Base.require_one_based_indexing(x)
     size(x) == p.in_size || throw(DimensionMismatch("plan and input array axes do not match"))
     y = p.dir == FFT_FORWARD ? Array{complex(T)}(undef, p.out_size) : Array{real(T)}(undef, p.out_size)
-    xs = x isa StridedArray ? x : collect(x)
+    xs = is_dense(x) ? x : Array(x)
     GC.@preserve xs y begin
         check(ccall((:fftahip_rplan_execute_host, libfftahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
                     p.handle, pointer(xs), pointer(y)))

@@ -94,6 +94,10 @@ fftahip_status fftahip_plan_check(fftahip_plan plan, int nregion, const int64_t*
 
 /* mul!(y, p, x) on device arrays of shape dims (src/plan.jl:157-167 for 1-D,
  * 170-207 for a 1-D plan on an N-D array, 210-257 and 260-325 for N-D).
+ * Applies mul!'s argument checks first (as fftahip_plan_check with the
+ * plan's region, x and y both of shape dims): a shape that does not fit the
+ * plan returns FFTAHIP_ERR_DIMENSION_MISMATCH before anything is launched.
+ * A pass over 2^31 or more lines is refused with FFTAHIP_ERR_CAPACITY.
  * Out of place; d_in may equal d_out.  Asynchronous on `stream`. */
 fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in, void* d_out,
                                void* stream);
@@ -180,7 +184,8 @@ fftahip_status fftahip_server_submit(fftahip_server server, int precision, int d
                                      const int64_t* dims, const void* in, void* out, int host_buffers,
                                      uint64_t* ticket);
 /* Submit `count` requests, request i at t_offset[i] seconds after the call
- * (a load generator for trace replay); dims holds 3 entries per request. */
+ * (a load generator for trace replay); dims holds 3 entries per request, so
+ * ndims[i] is 1 to 3 (else FFTAHIP_ERR_ARGUMENT, nothing submitted). */
 fftahip_status fftahip_server_replay(fftahip_server server, int64_t count, const double* t_offset,
                                      const int* precision, const int* direction, const int* ndims,
                                      const int64_t* dims, void* const* in, void* const* out, int host_buffers,

@@ -139,6 +139,52 @@ def test_mutated_region_is_invalid():
             p2 * x
 
 
+@pytest.mark.parametrize("shape,region,dims", [((64,), None, (65,)), ((64,), None, (32, 2)),
+                                                ((16, 8), None, (16, 9)), ((16, 8), 2, (16, 9)),
+                                                ((4, 4, 4), (1, 3), (4, 4)), ((4, 4), None, (5, 4, 4)),
+                                                ((8, 8), (1, 2), (8,))])
+def test_execute_checks_shape_before_any_launch(shape, region, dims):
+    """fftahip_execute and fftahip_execute_host apply mul!'s argument checks
+    themselves (src/plan.jl:157-164, 170-182, 210-222, 260-272): a shape that
+    does not fit the plan returns DimensionMismatch (status 1) before anything
+    is launched or copied.  The data pointers are deliberately bogus: they are
+    never touched."""
+    import ctypes
+
+    L = _lib.load()
+    p = P.Plan(shape, 64, region, -1)
+    bogus = ctypes.c_void_p(0x10)
+    d = _lib.i64_array(dims)
+    assert L.fftahip_execute(p.handle, len(dims), d, bogus, bogus, None) == _lib.ERR_DIMENSION_MISMATCH
+    msg = _lib.last_error()
+    assert msg.startswith(("plan has", "Plan region is outside", "Region is invalid", "input array")), msg
+    assert L.fftahip_execute_host(p.handle, len(dims), d, bogus, bogus) == _lib.ERR_DIMENSION_MISMATCH
+    with pytest.raises(P.DimensionMismatch):
+        p.execute_device(dims, 0x10, 0x10)
+
+
+def test_execute_rejects_negative_dims():
+    import ctypes
+
+    L = _lib.load()
+    p = P.Plan((8,), 32, None, -1)
+    bogus = ctypes.c_void_p(0x10)
+    assert L.fftahip_execute(p.handle, 1, _lib.i64_array([-8]), bogus, bogus, None) == _lib.ERR_ARGUMENT
+
+
+@pytest.mark.parametrize("shape,region", [((2, 2**32), 1), ((4, 2**31), 1), ((2**31, 2), 2), ((2, 2**31, 2), 3)])
+def test_line_count_capacity(shape, region):
+    """Line indices are 32-bit on the device: a pass over 2^31 or more lines is
+    refused with a capacity error at planning time instead of wrapping."""
+    with pytest.raises(fftahip.FFTAHipError, match="2\\^31"):
+        P.debug_schedule(shape, 32, -1, region)
+
+
+def test_line_count_below_capacity_plans():
+    s = P.debug_schedule((2, 2**30), 32, -1, 1)
+    assert s["launches"] and all(la["nlines"] < 2**31 for la in s["launches"])
+
+
 def test_describe_reports_ffta_tree():
     p = P.plan_fft(np.zeros(275, dtype=np.complex128))
     d = p.describe()
