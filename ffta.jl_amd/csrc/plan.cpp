@@ -1531,8 +1531,8 @@ fftahip_status fftahip_plan_create(fftahip_plan* plan, int precision, int direct
     if (st) return st;
   }
   // the schedule of the creation shape is built and uploaded now (with a
-  // device present), so that the first execution and a stream capture find
-  // it ready
+  // device present), so that the first execution, a stream capture and
+  // fftahip_execute_noalloc find it ready
   int64_t total = 1;
   for (int i = 0; i < ndims; ++i) total *= dims[i];
   if (!g_dry && total > 0 && dev_info().ok) {
@@ -1797,6 +1797,68 @@ fftahip_status fftahip_execute_host(fftahip_plan plan, int ndims, const int64_t*
   if ((st = fftahip_execute(plan, ndims, dims, plan->hin.p, plan->hout.p, nullptr))) return st;
   HIP_TRY(hipMemcpy(h_out, plan->hout.p, bytes, hipMemcpyDeviceToHost));
   return FFTAHIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ execution without allocation
+// fftahip_execute_noalloc takes its scratch from `scratch`, a fixed area of
+// device memory inside the library (every GPU the library runs on holds its
+// own copy until the process ends).  The array's name is a host-side handle,
+// not a device address: its address on the current device is looked up once
+// per device with hipGetSymbolAddress and only that pointer is passed to the
+// kernels.  Every pass writes each scratch element before any pass reads it,
+// so the area never needs clearing; calls that overlap on one GPU share it
+// and race.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static std::once_flag g_scratch_once[kMaxScratchDevices];
+static void* g_scratch_ptr[kMaxScratchDevices];
+static hipError_t g_scratch_err[kMaxScratchDevices];
+
+// Address of `scratch` on the current device (resolved once per device).
+static fftahip_status noalloc_scratch(void** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(FFTAHIP_ERR_CAPACITY, "device index " + std::to_string(dev) + " beyond the no-alloc table");
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* ptr = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? ptr : nullptr;
+  });
+  if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+    return fail(FFTAHIP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratch_err[dev]));
+  *out = g_scratch_ptr[dev];
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_execute_noalloc(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in,
+                                       void* d_out, void* stream) {
+  if (!plan) return fail(FFTAHIP_ERR_ARGUMENT, "NULL plan");
+  if (fftahip_status cs = check_shape(plan, ndims, dims)) return cs;
+  int64_t total = 1;
+  for (int i = 0; i < ndims; ++i) total *= dims[i];
+  if (total == 0) return FFTAHIP_SUCCESS;
+  // building a schedule uploads it (an allocation): only shapes whose
+  // schedule exists -- the creation shape, or one executed before -- run here
+  std::vector<int64_t> key(dims, dims + ndims);
+  auto it = plan->scheds.find(key);
+  if (it == plan->scheds.end())
+    return fail(FFTAHIP_ERR_NOT_READY,
+                "no schedule for array shape " + tuple_str(dims, ndims) +
+                    " on this plan: create the plan with this shape or run fftahip_execute on it once");
+  const DevSchedule& S = *it->second;
+  if (S.scratch_need > kNoAllocScratchBytes)
+    return fail(FFTAHIP_ERR_CAPACITY, "this transform needs " + std::to_string(S.scratch_need) +
+                                          " bytes of scratch; fftahip_execute_noalloc has " +
+                                          std::to_string(kNoAllocScratchBytes));
+  void* scr = nullptr;
+  if (S.scratch_need > 0) {
+    fftahip_status st = noalloc_scratch(&scr);
+    if (st) return st;
+  }
+  return run_schedule(S, plan->prec, BufPtrs{d_in, d_out, scr}, (hipStream_t)stream);
 }
 
 // ------------------------------------------------------------------ real transforms

@@ -41,6 +41,7 @@ SIGNATURES = {
     "fftahip_plan_check": (_st, [_vp, ctypes.c_int, _i64p, ctypes.c_int, _i64p, ctypes.c_int, _i64p]),
     "fftahip_execute": (_st, [_vp, ctypes.c_int, _i64p, _vp, _vp, _vp]),
     "fftahip_execute_host": (_st, [_vp, ctypes.c_int, _i64p, _vp, _vp]),
+    "fftahip_execute_noalloc": (_st, [_vp, ctypes.c_int, _i64p, _vp, _vp, _vp]),
     "fftahip_plan_describe": (_st, [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "fftahip_plan_destroy": (_st, [_vp]),
     "fftahip_rplan_create": (_st, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p,

@@ -107,6 +107,21 @@ fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims
 fftahip_status fftahip_execute_host(fftahip_plan plan, int ndims, const int64_t* dims, const void* h_in,
                                     void* h_out);
 
+/* fftahip_execute without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  It
+ * calls no hipMalloc/hipFree or other allocating routine; its scratch is a
+ * fixed 64 MiB area inside the library on each GPU, held until the process
+ * ends.  Limits: a transform needing more scratch than that fails with
+ * FFTAHIP_ERR_CAPACITY; the array shape must already have a schedule on the
+ * plan (the shape the plan was created with, or one run through
+ * fftahip_execute before), else FFTAHIP_ERR_NOT_READY.  All calls on one GPU
+ * share the area, which keeps the last call's contents: calls that overlap
+ * on one GPU (concurrent streams or threads) race.  Same argument checks and
+ * results as fftahip_execute. */
+fftahip_status fftahip_execute_noalloc(fftahip_plan plan, int ndims, const int64_t* dims, const void* d_in,
+                                       void* d_out, void* stream);
+
 /* JSON description: FFTA's call-graph per region dimension (node type, size,
  * strides) and the device schedule (passes, radices, tiles). */
 fftahip_status fftahip_plan_describe(fftahip_plan plan, char* buf, size_t cap, size_t* needed);
