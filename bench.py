@@ -397,6 +397,8 @@ def main(argv=None) -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="tune the workload's in-LDS radix schedules on this device, write TUNING_DB and exit")
     ap.add_argument("--no-tuning", action="store_true", help="ignore TUNING_DB (default radix rules)")
+    ap.add_argument("--cutoff", type=int, default=73,
+                    help="BLUESTEIN_CUTOFF of the batch (FFTA's keyword; default 73 as FFTA)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="after the steps, copy 1 GiB once (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args(argv)
@@ -422,7 +424,7 @@ def main(argv=None) -> int:
     if not args.no_tuning and os.path.exists(TUNING_DB):
         tuned_entries = autotune.load_table(TUNING_DB)
     t0 = time.perf_counter()
-    batch = Batch(lengths, dirs, precision=32)
+    batch = Batch(lengths, dirs, precision=32, cutoff=args.cutoff)
     plan_s = time.perf_counter() - t0
     total = int(lengths.sum())
     gen = torch.Generator(device="cuda")
@@ -487,6 +489,7 @@ def main(argv=None) -> int:
             "launch": "captured hipGraph (serial chain)" if args.graph else "launch DAG on 4 streams (eager)",
             ("eager_dag_ms_per_step" if args.graph else "graph_ms_per_step"): round(other_ms, 4),
             "autotuned_radix_schedules": tuned_entries,
+            "bluestein_cutoff": args.cutoff,
         },
         "roofline": {
             "bound": "hbm",

@@ -199,21 +199,125 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
   __syncthreads();
 }
 
+// One radix-P stage for a prime P >= 17 over a line in LDS: the direct DFT
+// of P points (FFTA runs primes below BLUESTEIN_CUTOFF as O(P^2) DFT leaves,
+// src/algos.jl:135-160), in Stockham order like stage().  With
+// s_j = x_j + x_{P-j} and d_j = x_j - x_{P-j} (j = 1..H, H = (P-1)/2):
+//   X_0 = x_0 + sum_j s_j,
+//   X_k = A_k - i B_k,  X_{P-k} = A_k + i B_k,
+//   A_k = x_0 + sum_j cos(2 pi jk/P) s_j,  B_k = sum_j sin(2 pi jk/P) d_j.
+// A work item is (butterfly j, group g of kBigKP consecutive k); butterflies
+// run fastest across threads, and a thread holds at most kBigHold work
+// items across the barrier (the planner sizes Tl for that).  `roots` holds
+// w_P^m = exp(-2 pi i m/P), m < P, in the pass precision; the exponent j*k
+// mod P advances by k per step.  Stage twiddles (Ns > 1) are applied to the
+// inputs as they are read, from the same per-stage table as stage().
+constexpr int kBigKP = 4;
+constexpr int kBigHold = 2;
+
+template <class Rt>
+__device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int Tl, int L, int P, int Ns, FastDiv nsd,
+                                          const cx<Rt>* __restrict__ tw, const cx<Rt>* __restrict__ roots, int sh) {
+  const int Lr = L / P;
+  const int H = (P - 1) >> 1;
+  const int G = (H + kBigKP - 1) / kBigKP;
+  const int nwork = Lr * G;
+  const int she = pad_shift(sh);
+  cx<Rt> out[kBigHold][2 * kBigKP + 1];
+  int obase[kBigHold], og[kBigHold];
+#pragma unroll
+  for (int h = 0; h < kBigHold; ++h) {
+    const int w = jl + h * Tl;
+    og[h] = -1;
+    obase[h] = 0;
+    if (!(active && w < nwork)) continue;
+    const int g = w / Lr;
+    const int j = w - g * Lr;
+    const uint32_t q = fdiv((uint32_t)j, nsd);
+    const int k = j - (int)q * Ns;
+    const cx<Rt>* __restrict__ wt = tw + (Ns - 1) + k * (P - 1);
+    const cx<Rt> x0 = line[pad_idx(j, she)];
+    cx<Rt> A[kBigKP], B[kBigKP];
+    int m[kBigKP];
+#pragma unroll
+    for (int u = 0; u < kBigKP; ++u) {
+      A[u] = x0;
+      B[u] = {(Rt)0, (Rt)0};
+      m[u] = 0;
+    }
+    cx<Rt> sum = x0;
+    const int k0 = g * kBigKP + 1;
+    for (int i = 1; i <= H; ++i) {
+      cx<Rt> a = line[pad_idx(j + i * Lr, she)];
+      cx<Rt> b = line[pad_idx(j + (P - i) * Lr, she)];
+      if (Ns > 1) {
+        a = cmul(a, gld(wt + (i - 1)));
+        b = cmul(b, gld(wt + (P - i - 1)));
+      }
+      const cx<Rt> s = cadd(a, b), d = csub(a, b);
+      sum = cadd(sum, s);
+#pragma unroll
+      for (int u = 0; u < kBigKP; ++u) {
+        m[u] += k0 + u;
+        if (m[u] >= P) m[u] -= P;
+        const cx<Rt> r = gld(roots + m[u]);  // (cos, -sin) of 2 pi m / P
+        A[u].x += r.x * s.x;
+        A[u].y += r.x * s.y;
+        B[u].x -= r.y * d.x;
+        B[u].y -= r.y * d.y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBigKP; ++u) {
+      out[h][2 * u] = {A[u].x + B[u].y, A[u].y - B[u].x};
+      out[h][2 * u + 1] = {A[u].x - B[u].y, A[u].y + B[u].x};
+    }
+    out[h][2 * kBigKP] = sum;
+    og[h] = g;
+    obase[h] = (int)(q * (uint32_t)Ns * (uint32_t)P) + k;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < kBigHold; ++h) {
+    if (og[h] < 0) continue;
+    const int base = obase[h];
+#pragma unroll
+    for (int u = 0; u < kBigKP; ++u) {
+      const int kk = og[h] * kBigKP + 1 + u;
+      if (kk > H) break;
+      line[pad_idx(base + kk * Ns, she)] = out[h][2 * u];
+      line[pad_idx(base + (P - kk) * Ns, she)] = out[h][2 * u + 1];
+    }
+    if (og[h] == 0) line[pad_idx(base, she)] = out[h][2 * kBigKP];
+  }
+  __syncthreads();
+}
+
 // All radix stages of a line.  With fuse_first the first stage reads its
 // inputs through ld(element) from global memory instead of LDS; with
 // fuse_last the last stage writes through st(element, value).
-template <class Rt, int EPT, bool FUSE, class LdF, class StF>
+template <class Rt, int EPT, bool FUSE, bool BIG, class LdF, class StF>
 __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const LinePass& p,
                                       const cx<Rt>* __restrict__ tw, bool fuse_first, bool fuse_last, const LdF& ld,
                                       const StF& st, const cx<Rt>* __restrict__ pm = nullptr, int64_t pms = 0) {
   int Ns = 1;
   const int sh = p.lds_sh;
   const int nst = p.nst;
+  // roots of the radix-P stages with P >= 17 follow the L - 1 stage entries
+  int roots_off = p.L - 1;
   for (int s = 0; s < nst; ++s) {
     const int r = p.rad[s];
     const FastDiv nsd = p.nsdiv[s];
     const bool gl = fuse_first && s == 0;
     const bool gs = fuse_last && s == nst - 1;
+    if (BIG && r >= 17) {
+      // the planner keeps fused global access and the in-stage spectrum
+      // product off stages of this kind
+      stage_big<Rt>(line, jl, active, p.Tl, p.L, r, Ns, nsd, tw, tw + roots_off, sh);
+      roots_off += r;
+      Ns *= r;
+      continue;
+    }
     switch (r) {
 #define FFTAHIP_STAGE(R) \
   case R: stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == 0 ? pm : nullptr, pms); break;
@@ -344,7 +448,7 @@ __device__ __forceinline__ double2 step_twiddle(const LinePass& p, int64_t twk) 
   return dmul2(gld(p.tw_lo + (m & mask)), gld(p.tw_hi + (m >> p.tw_sh)));
 }
 
-template <class Rt, int EPT, int BLOCK, int MODE>
+template <class Rt, int EPT, int BLOCK, int MODE, bool BIG>
 __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& bp, uint32_t tile, cx<Rt>* lds,
                                               const LineShared& sh) {
   constexpr int U = Batch<MODE>::U;
@@ -546,10 +650,10 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   };
   // only the plain variant carries the fused global paths (register cost)
   constexpr bool kFuse = MODE == MODE_PLAIN;
-  sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, fuse_in, fuse_out && !has_mid, ld, st);
+  sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, fuse_in, fuse_out && !has_mid, ld, st);
 
   if (has_mid && p.mid_in_stage) {
-    sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, false, fuse_out, ld, st, bhat + sh.cmid[cs], p.cme);
+    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, false, fuse_out, ld, st, bhat + sh.cmid[cs], p.cme);
   } else if (has_mid) {
     if (p.row_mid && col_step) {
       if (row_c < nvalid) {
@@ -595,7 +699,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         if (slot[u] >= 0) lds[slot[u]] = cconj(cmul(lds[slot[u]], f[u]));
     }
     __syncthreads();
-    sweep<Rt, EPT, kFuse>(myline, jl, active, p, tw, false, fuse_out, ld, st);
+    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, false, fuse_out, ld, st);
   }
 
   // ---- store (LDS -> global).  As on the load side, the table factors of a
@@ -755,8 +859,12 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
 // L2 the neighbouring lines).  At least 4 waves per SIMD (<= 128 VGPRs), 7
 // for the FP32 plain variant (<= 72 VGPRs): the passes are latency bound on
 // global memory.
-template <class Rt, int EPT, int BLOCK, int MODE>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MODE == MODE_PLAIN && sizeof(Rt) == 4 ? 7 : 4)))
+// BIG: the variant whose sweeps also run direct-DFT stages of prime radices
+// >= 17 (stage_big); kept apart so that the other variants carry none of its
+// register cost.
+template <class Rt, int EPT, int BLOCK, int MODE, bool BIG>
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(MODE == MODE_PLAIN && sizeof(Rt) == 4 && !BIG ? 7 : 4)))
 line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __restrict__ tile_start, int ngroups,
                        BufPtrs bp, uint32_t ntiles) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -772,7 +880,7 @@ line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __re
   const int C = p.C;
   int64_t* tab = reinterpret_cast<int64_t*>(smem_raw + (tile_lds_bytes(C, p.ldl, sizeof(cx<Rt>)) - (size_t)C * kLineTabBytes));
   const LineShared sh{tab, tab + C, tab + 2 * C, tab + 3 * C, tab + 4 * C, tab + 5 * C};
-  run_line_pass<Rt, EPT, BLOCK, MODE>(p, bp, bid - tile_start[lo], reinterpret_cast<cx<Rt>*>(smem_raw), sh);
+  run_line_pass<Rt, EPT, BLOCK, MODE, BIG>(p, bp, bid - tile_start[lo], reinterpret_cast<cx<Rt>*>(smem_raw), sh);
 }
 
 // ------------------------------------------------------------------ register line passes

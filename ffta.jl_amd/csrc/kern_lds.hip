@@ -1,8 +1,9 @@
 // kern_lds.hip -- one instantiation of the in-LDS line-pass kernel per
 // (precision, workgroup size), so that the variants compile in parallel.
 // Built once per variant with -DKRT=float|double -DKTAG=f32|f64
-// -DKBLOCK=256|512|1024; exports lds_fn_<KTAG>_<KBLOCK>(mode), the kernel
-// entry of each pass mode (plain, pre-twiddle, full).
+// -DKBLOCK=256|512|1024; exports lds_fn_<KTAG>_<KBLOCK>(variant), the kernel
+// entry of each pass variant (plain, pre-twiddle, full; each with and
+// without direct-DFT stages of prime radices >= 17).
 #include "device.h"
 #include "kernels.h"
 
@@ -15,10 +16,19 @@
 
 namespace fftahip {
 
-const void* FFTAHIP_CAT4(lds_fn_, KTAG, _, KBLOCK)(int mode) {
-  if (mode == MODE_PLAIN) return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_PLAIN>);
-  if (mode == MODE_PRETW) return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_PRETW>);
-  return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_FULL>);
+const void* FFTAHIP_CAT4(lds_fn_, KTAG, _, KBLOCK)(int variant) {
+  switch (variant) {
+    case MODE_PLAIN: return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_PLAIN, false>);
+    case MODE_PRETW: return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_PRETW, false>);
+    case MODE_FULL: return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_FULL, false>);
+    case kBigVariant + MODE_PLAIN:
+      return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_PLAIN, true>);
+    case kBigVariant + MODE_PRETW:
+      return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_PRETW, true>);
+    case kBigVariant + MODE_FULL:
+      return reinterpret_cast<const void*>(&line_pass_group_kernel<KRT, 16, KBLOCK, MODE_FULL, true>);
+    default: return nullptr;
+  }
 }
 
 }  // namespace fftahip

@@ -14,7 +14,7 @@ struct KernelRef {
   const void* fn;
   int block;
 };
-KernelRef lds_kernel(int precision, int block, int mode);
+KernelRef lds_kernel(int precision, int block, int variant);
 KernelRef reg_kernel(int precision, int mode);
 // Kernel arguments of a work-list launch (passes, tile starts, group count,
 // buffers, tile count), laid out for hipLaunchKernel / kernel nodes.
@@ -40,10 +40,16 @@ const void* lds_fn_f64_512(int mode);
 const void* lds_fn_f64_1024(int mode);
 const void* reg_fn_f32(int mode);
 const void* reg_fn_f64(int mode);
-// Kernel variant a pass needs (0 plain, 1 load-side pass twiddle, 2 full).
+// Kernel mode a pass needs (0 plain, 1 load-side pass twiddle, 2 full).
 int pass_mode(const LinePass& p);
+// In-LDS kernel variant of a pass: its mode, plus kBigVariant when a stage
+// of its radix schedule is a prime >= 17 (the direct-DFT stage).
+constexpr int kBigVariant = 3;
+constexpr int kLdsVariants = 6;
+int pass_variant(const LinePass& p);
 // Launch ngroups independent passes (device descriptors) as one grid.
-hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
+// `variant`: pass_variant of every group of the launch.
+hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int variant,
                                    int precision, int block, const BufPtrs& bp, int remap, size_t lds_bytes,
                                    int64_t grid, hipStream_t stream);
 // Register line passes (one short line per thread; lengths are the 13-smooth
@@ -51,7 +57,8 @@ hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_t
 int reg_pass_max(int precision, int mode);
 hipError_t launch_reg_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
                                   int precision, const BufPtrs& bp, int64_t grid, hipStream_t stream);
-hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t L, hipStream_t stream);
+// n = table entries: L - 1 stage twiddles, then P roots per stage of a prime radix P >= 17.
+hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t n, hipStream_t stream);
 hipError_t launch_gen_twiddle_2level(double2* lo, double2* hi, int64_t N, int sh, int64_t nhi, hipStream_t stream);
 hipError_t launch_gen_tw16(void* out, int precision, int64_t N, int64_t s, int64_t amax, hipStream_t stream);
 // Real-transform boundary kernels (see kernels.hip); lo/hi/sh are the

@@ -22,10 +22,12 @@ namespace fftahip {
 
 // Per-stage twiddles of a radix schedule: stage s (Ns = product of earlier
 // radices, radix R) holds w_{Ns*R}^{t*k} for k < Ns, t = 1..R-1 at
-// (Ns-1) + k*(R-1) + t-1; L-1 entries in all.
+// (Ns-1) + k*(R-1) + t-1; L-1 entries in all.  After them, every stage of a
+// prime radix P >= 17 (stage_big) appends the P roots w_P^m, m < P.
 template <class Rt>
-__global__ void gen_stage_twiddles_kernel(cx<Rt>* tw, RadixList rl) {
+__global__ void gen_stage_twiddles_kernel(cx<Rt>* tw, RadixList rl, int64_t n) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
   int64_t Ns = 1;
   for (int s = 0; s < rl.nst; ++s) {
     const int64_t R = rl.rad[s];
@@ -37,6 +39,16 @@ __global__ void gen_stage_twiddles_kernel(cx<Rt>* tw, RadixList rl) {
       return;
     }
     Ns *= R;
+  }
+  int64_t off = Ns - 1;  // Ns = L here
+  for (int s = 0; s < rl.nst; ++s) {
+    const int64_t R = rl.rad[s];
+    if (R < 17) continue;
+    if (m >= off && m < off + R) {
+      tw[m] = from_d2<Rt>(unit_root(m - off, R));
+      return;
+    }
+    off += R;
   }
 }
 
@@ -272,6 +284,13 @@ int pass_mode(const LinePass& p) {
   return p.pretw == 1 ? MODE_PRETW : MODE_PLAIN;
 }
 
+int pass_variant(const LinePass& p) {
+  int big = 0;
+  for (int s = 0; s < p.nst; ++s)
+    if (p.rad[s] >= 17) big = kBigVariant;
+  return pass_mode(p) + big;
+}
+
 KernelRef lds_kernel(int precision, int block, int mode) {
   KernelRef k{nullptr, block};
   if (precision == 32) {
@@ -293,8 +312,8 @@ KernelRef reg_kernel(int precision, int mode) {
 void init_kernels(size_t lds_per_block) {
   for (int prec : {32, 64})
     for (int block : {256, 512, 1024})
-      for (int mode : {(int)MODE_PLAIN, (int)MODE_PRETW, (int)MODE_FULL})
-        (void)hipFuncSetAttribute(lds_kernel(prec, block, mode).fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+      for (int variant = 0; variant < kLdsVariants; ++variant)
+        (void)hipFuncSetAttribute(lds_kernel(prec, block, variant).fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_per_block);
   (void)hipGetLastError();
 }
@@ -312,11 +331,11 @@ LaunchArgs::LaunchArgs(const LinePass* P, const uint32_t* T, int ng, const BufPt
 
 uint32_t padded_grid(int64_t grid) { return (uint32_t)((grid + 7) & ~7ll); }
 
-hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int mode,
+hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int variant,
                                    int precision, int block, const BufPtrs& bp, int remap, size_t lds_bytes,
                                    int64_t grid, hipStream_t stream) {
   (void)remap;
-  const KernelRef k = lds_kernel(precision, block, mode);
+  const KernelRef k = lds_kernel(precision, block, variant);
   if (!k.fn) return hipErrorInvalidValue;
   LaunchArgs a(d_passes, d_tile_start, ngroups, bp, grid);
   return hipLaunchKernel(k.fn, dim3(padded_grid(grid)), dim3(k.block), a.args, lds_bytes, stream);
@@ -338,14 +357,14 @@ int reg_pass_max(int precision, int mode) {
 }
 
 
-hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t L, hipStream_t stream) {
-  const int64_t blocks = (L + 255) / 256;
+hipError_t launch_gen_stage_twiddles(void* tw, int precision, const RadixList& rl, int64_t n, hipStream_t stream) {
+  const int64_t blocks = (n + 255) / 256;
   if (precision == 32)
     hipLaunchKernelGGL(gen_stage_twiddles_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                       reinterpret_cast<cx<float>*>(tw), rl);
+                       reinterpret_cast<cx<float>*>(tw), rl, n);
   else
     hipLaunchKernelGGL(gen_stage_twiddles_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                       reinterpret_cast<cx<double>*>(tw), rl);
+                       reinterpret_cast<cx<double>*>(tw), rl, n);
   return hipGetLastError();
 }
 

@@ -214,16 +214,39 @@ struct Inner {
   int block = 256;
 };
 
+// Radices of the in-register codelets (codelets.h); any other radix is a
+// prime P >= 17 run by the direct-DFT stage (stage_big in device.h).
+static bool codelet_radix(int r) {
+  switch (r) {
+    case 2: case 3: case 4: case 5: case 7: case 8: case 9: case 11: case 13: case 16: return true;
+    default: return false;
+  }
+}
+
+// Largest prime the direct-DFT stage takes (larger primes go through a
+// convolution whatever BLUESTEIN_CUTOFF says).
+constexpr int64_t kMaxDirectPrime = 1021;
+// Direct-DFT stage geometry (device.h): work items of kBigKP output pairs,
+// kBigHold per thread.
+constexpr int64_t kBigKPHost = 4, kBigHoldHost = 2;
+
 // In-LDS line schedule with the given radices (in stage order): threads per
-// line Tl = max over stages of ceil((L / R) / (16 / R)).
+// line Tl = max over stages of ceil((L / R) / (16 / R)) for codelet radices,
+// and for a direct-DFT stage of prime P the work items over kBigHold:
+// ceil((L / P) * ceil(((P - 1) / 2) / kBigKP) / kBigHold).
 static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in) {
   if ((int)rad.size() > kMaxStages) return false;
   int64_t prod = 1, Tl = 1;
   for (int r : rad) {
-    if (r < 2 || r > kEPT) return false;
+    if (codelet_radix(r)) {
+      const int64_t nb = kEPT / r;
+      Tl = std::max(Tl, ((L / r) + nb - 1) / nb);
+    } else {
+      if (r < 17 || r > kMaxDirectPrime || !is_prime(r)) return false;
+      const int64_t groups = ((r - 1) / 2 + kBigKPHost - 1) / kBigKPHost;
+      Tl = std::max(Tl, ((L / r) * groups + kBigHoldHost - 1) / kBigHoldHost);
+    }
     prod *= r;
-    const int64_t nb = kEPT / r;
-    Tl = std::max(Tl, ((L / r) + nb - 1) / nb);
   }
   if (prod != L || Tl > 1024) return false;
   in = Inner();
@@ -238,11 +261,35 @@ static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in) {
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, int64_t>, std::vector<int>> g_tuned;
 
-// Radix schedule for an in-LDS line of length L (prime factors <= 13): the
-// autotuned one for (prec, L) if there is one, else the default rule --
-// 13, 11, 7, 5 as single stages, powers of two in as few stages as possible
-// with sizes balanced (each <= 16), 9s then a 3, largest radix first.
-static bool make_inner(int64_t L, Inner& in, int prec = 0) {
+// Primes a length may contain and still run as direct DFT stages: 2 and 3
+// always (FFTA classifies powers of 2 and 3 before primes), any other prime
+// p when p < BLUESTEIN_CUTOFF (src/callgraph.jl:95-100) -- codelets up to 13,
+// the direct-DFT stage from 17 up to kMaxDirectPrime.  Internal lengths (the
+// convolution lengths of Bluestein and Rader) use kInternalCutoff.
+constexpr int64_t kInternalCutoff = 73;
+static bool direct_prime(int64_t p, int64_t cutoff) {
+  if (p <= 3) return true;
+  return p < cutoff && p <= kMaxDirectPrime;
+}
+
+// True when every prime factor of n is a direct prime for `cutoff`.
+static bool direct_smooth(int64_t n, int64_t cutoff) {
+  for (int64_t p = 2; p * p <= n; ++p) {
+    if (n % p) continue;
+    if (!direct_prime(p, cutoff)) return false;
+    while (n % p == 0) n /= p;
+  }
+  return n == 1 || direct_prime(n, cutoff);
+}
+
+// Radix schedule for an in-LDS line of length L whose primes are direct for
+// `cutoff`: the autotuned one for (prec, L) if there is one, else the
+// default rule -- primes >= 17 as direct-DFT stages, 13, 11, 7, 5 as single
+// stages, powers of two in as few stages as possible with sizes balanced
+// (each <= 16), 9s then a 3, largest radix first (so a direct-DFT stage
+// comes first and needs no stage twiddles).
+static bool make_inner(int64_t L, Inner& in, int prec = 0, int64_t cutoff = kInternalCutoff) {
+  if (L < 1 || !direct_smooth(L, cutoff)) return false;
   if (prec) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tuned.find({prec, L});
@@ -255,7 +302,12 @@ static bool make_inner(int64_t L, Inner& in, int prec = 0) {
   std::vector<int> rad;
   for (int p : {13, 11, 7, 5})
     while (m % p == 0) { m /= p; rad.push_back(p); }
-  if (m != 1) return false;
+  for (int64_t p = 17; m > 1 && p * p <= m; p += 2)
+    while (m % p == 0) { m /= p; rad.push_back((int)p); }
+  if (m > 1) {
+    if (m < 17) return false;
+    rad.push_back((int)m);
+  }
   // powers of two: as few stages as possible, sizes balanced, each <= 16
   if (e2 > 0) {
     int ns = (e2 + 3) / 4;
@@ -393,7 +445,11 @@ static fftahip_status stage_table(Tables& T, const Inner& in, const void** out, 
   auto it = T.stage.find(key);
   if (it == T.stage.end()) {
     auto buf = std::make_unique<DevBuf>();
-    const int64_t n = std::max<int64_t>(1, in.L - 1);
+    // L - 1 stage twiddles, then the P roots of each direct-DFT stage
+    int64_t n = in.L - 1;
+    for (int r : in.rad)
+      if (!codelet_radix(r)) n += r;
+    n = std::max<int64_t>(1, n);
     fftahip_status st = buf->ensure((size_t)n * csize(T.prec));
     if (st) return st;
     RadixList rl;
@@ -526,7 +582,7 @@ static bool blue3_split(int64_t M, int prec, int64_t* M1, int64_t* M2) {
 }
 
 // Split M into pass factors, each <= cap and smooth; balanced greedily.
-static bool split_factors(int64_t M, int64_t cap, std::vector<int64_t>& F) {
+static bool split_factors(int64_t M, int64_t cap, std::vector<int64_t>& F, int64_t cutoff = kInternalCutoff) {
   std::vector<int64_t> primes = prime_factors(M);
   std::sort(primes.begin(), primes.end(), std::greater<int64_t>());
   for (int p = 2; p <= 8; ++p) {
@@ -547,7 +603,7 @@ static bool split_factors(int64_t M, int64_t cap, std::vector<int64_t>& F) {
     // every factor must have an in-LDS schedule
     bool sched = true;
     Inner tmp;
-    for (int64_t x : F) sched = sched && make_inner(x, tmp);
+    for (int64_t x : F) sched = sched && make_inner(x, tmp, 0, cutoff);
     if (sched) return true;
   }
   return false;
@@ -561,8 +617,9 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
     d.kind = K_MIXED;
     d.M = lp.N;
     std::vector<int64_t> FS;
-    if (lp.S <= pass_max(prec)) FS = {lp.S};
-    else if (!split_factors(lp.S, pass_max(prec), FS))
+    Inner tmp;
+    if (lp.S <= pass_max(prec) && make_inner(lp.S, tmp, prec, lp.cutoff)) FS = {lp.S};
+    else if (!split_factors(lp.S, pass_max(prec), FS, lp.cutoff))
       return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(lp.S));
     d.F = {lp.bq};
     Inner qb;
@@ -570,18 +627,20 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
     d.inner = {qb};
     for (int64_t f : FS) {
       Inner in;
-      make_inner(f, in, prec);
+      make_inner(f, in, prec, lp.cutoff);
       d.F.push_back(f);
       d.inner.push_back(in);
     }
     return FFTAHIP_SUCCESS;
   }
   const int64_t target = lp.blue ? lp.M : lp.N;
+  // the user length's own primes follow the plan's cutoff; Bluestein's M is internal
+  const int64_t cut = lp.blue ? kInternalCutoff : lp.cutoff;
   d.M = target;
   const int64_t single_cap = strided ? pass_max(prec) : single_max(prec);
   if (target <= single_cap) {
     Inner in;
-    if (make_inner(target, in, prec)) {
+    if (make_inner(target, in, prec, cut)) {
       d.kind = lp.blue ? K_BLUE_SINGLE : K_SINGLE;
       d.F = {target};
       d.inner = {in};
@@ -599,13 +658,13 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
     return FFTAHIP_SUCCESS;
   }
   std::vector<int64_t> F;
-  if (!split_factors(target, pass_max(prec), F))
+  if (!split_factors(target, pass_max(prec), F, cut))
     return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(target));
   d.kind = lp.blue ? K_BLUE_MULTI : K_MULTI;
   d.F = F;
   for (int64_t f : F) {
     Inner in;
-    make_inner(f, in, prec);
+    make_inner(f, in, prec, cut);
     d.inner.push_back(in);
   }
   return FFTAHIP_SUCCESS;
@@ -771,7 +830,7 @@ static double lds_cost(const Inner& in, int C, int ldl, int sh, int words, bool 
 static bool reg_eligible(const LinePass& p, const Inner& in, int prec) {
   if (env_i64("FFTAHIP_REG", 1) == 0) return false;
   if (p.mid_mode != MID_NONE || !p.col_in || !p.col_out) return false;
-  if (in.L < 2 || in.L > reg_pass_max(prec, pass_mode(p))) return false;
+  if (in.L < 2 || in.L > reg_pass_max(prec, pass_mode(p)) || !smooth13(in.L)) return false;
   return p.n[0] * (int64_t)csize(prec) >= 128;
 }
 
@@ -862,8 +921,11 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
   // and the pass has no table-driven element operations
   const bool plain = p.load_mode == LOAD_PLAIN && p.mid_mode == MID_NONE && p.store_mode == STORE_PLAIN && p.pretw == 0;
   const bool fuse = env_i64("FFTAHIP_FUSE", 1) != 0 && !in.rad.empty() && plain && in.Tl * esz >= 128;
-  p.fuse_in = (fuse && !p.col_in) ? 1 : 0;
-  p.fuse_out = (fuse && !p.col_out) ? 1 : 0;
+  // the direct-DFT stage reads and writes LDS only
+  const bool big_first = !in.rad.empty() && !codelet_radix(in.rad.front());
+  const bool big_last = !in.rad.empty() && !codelet_radix(in.rad.back());
+  p.fuse_in = (fuse && !p.col_in && !big_first) ? 1 : 0;
+  p.fuse_out = (fuse && !p.col_out && !big_last) ? 1 : 0;
   const int64_t threads = C * in.Tl;
   p.C = (int32_t)C;
   p.ldl = (int32_t)ldl;
@@ -879,7 +941,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
     p.row_in = (wide && !p.col_in && !p.fuse_in) ? 1 : 0;
     p.row_out = (wide && !p.col_out && !p.fuse_out) ? 1 : 0;
     p.row_mid = wide ? 1 : 0;
-    p.mid_in_stage = (p.mid_mode == MID_BHAT && env_i64("FFTAHIP_MID_STAGE", 1) != 0) ? 1 : 0;
+    p.mid_in_stage = (p.mid_mode == MID_BHAT && env_i64("FFTAHIP_MID_STAGE", 1) != 0 && !big_first) ? 1 : 0;
   }
   L.lds = tile_lds_bytes(C, ldl, esz);
   L.grid = (p.nlines + C - 1) / C;
@@ -1243,7 +1305,7 @@ static fftahip_status run_schedule(const DevSchedule& S, int prec, const BufPtrs
                                      prec, bp, la.grid, s));
       continue;
     }
-    HIP_TRY(launch_line_pass_groups((const LinePass*)S.passes.p + i, (const uint32_t*)S.zero.p, 1, pass_mode(la.lp),
+    HIP_TRY(launch_line_pass_groups((const LinePass*)S.passes.p + i, (const uint32_t*)S.zero.p, 1, pass_variant(la.lp),
                                     prec, la.block, bp, la.lp.xcd_remap, la.lds, la.grid, s));
   }
   return FFTAHIP_SUCCESS;
@@ -1328,7 +1390,10 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
   lp.cutoff = cutoff;
   lp.ffta.clear();
   ffta_nodes(lp.ffta, N, cutoff, 1, 1);
-  lp.blue = !smooth13(N);
+  // primes p < cutoff (and 2, 3) run as direct DFT stages, as FFTA runs
+  // primes below BLUESTEIN_CUTOFF as DFT leaves (src/callgraph.jl:95-100);
+  // any other prime factor goes through a convolution
+  lp.blue = !direct_smooth(N, cutoff);
   lp.mixed = false;
   lp.S = 1;
   lp.bq = N;
@@ -1342,9 +1407,14 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
     // LDS, run Q as a Bluestein pass and S as Stockham passes (FFTA splits
     // such lengths the same way, src/callgraph.jl:102-124, with Bluestein at
     // the prime leaf); otherwise Bluestein on the whole length.
-    int64_t S = 1, rest = N;
-    for (int q : {2, 3, 5, 7, 11, 13})
-      while (rest % q == 0) { rest /= q; S *= q; }
+    int64_t S = 1, rest = N, m = N;
+    for (int64_t q = 2; q * q <= m; ++q) {
+      if (m % q) continue;
+      int64_t pk = 1;
+      while (m % q == 0) { m /= q; pk *= q; }
+      if (direct_prime(q, cutoff)) { S *= pk; rest /= pk; }
+    }
+    if (m > 1 && direct_prime(m, cutoff)) { S *= m; rest /= m; }
     if (S > 1) {
       const int64_t M = best_line_length(rest, prec);
       if (M) {
@@ -1453,7 +1523,7 @@ struct fftahip_batch_s {
   struct Phase {
     bool reg;  // register line passes (block kRegBlock, no LDS)
     int block;
-    int mode;
+    int mode;  // pass_mode for register phases, pass_variant for in-LDS phases
     size_t lds;
     int64_t grid;
     int ngroups;
@@ -2374,7 +2444,9 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
       P.grid = tiles;
       b->phases.push_back(P);
     }
-    for (int mode = 2; mode >= 0; --mode)
+    // in-LDS passes: one launch per (kernel variant, block, LDS class); the
+    // full variants first, as above
+    for (int mode : {kBigVariant + 2, 2, kBigVariant + 1, 1, kBigVariant, 0})
     for (int block : {256, 512, 1024}) {
       for (size_t cls : kLdsClass) {
         fftahip_batch_s::Phase P;
@@ -2389,7 +2461,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
         for (size_t g = 0; g < per_group.size(); ++g) {
           auto& L = per_group[g];
           if (ph >= L.size() || L[ph].reg || L[ph].block != block || L[ph].grid <= 0 ||
-              lds_class(L[ph].lds) != cls || pass_mode(L[ph].lp) != mode)
+              lds_class(L[ph].lds) != cls || pass_variant(L[ph].lp) != mode)
             continue;
           LinePass lp = L[ph].lp;
           lp.xcd_remap = 0;

@@ -17,6 +17,10 @@ import numpy as np
 TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB, TK_TW16, \
     TK_CHIRPD = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12
 EPT = 16
+CODELETS = {2, 3, 4, 5, 7, 8, 9, 11, 13, 16}
+# direct-DFT stage of a prime radix P >= 17 (device.h stage_big): work items
+# of BIG_KP output pairs, BIG_HOLD per thread
+BIG_KP, BIG_HOLD = 4, 2
 LINE_TAB_BYTES = 6 * 8  # per-line offset tables after a tile's lines
 # register line passes: lines per workgroup, and the lengths with a codelet
 REG_BLOCK = 256
@@ -46,14 +50,62 @@ def stage_table(L: int, rad) -> np.ndarray:
     """Per-stage twiddles as the device table holds them: stage s (Ns = product
     of earlier radices, radix R) stores w_{Ns*R}^{t*k}, k < Ns, t = 1..R-1, at
     (Ns-1) + k*(R-1) + t-1."""
-    tab = np.zeros(max(1, L - 1), dtype=np.complex128)
+    extra = sum(R for R in rad if R not in CODELETS)
+    tab = np.zeros(max(1, L - 1 + extra), dtype=np.complex128)
     Ns = 1
     for R in rad:
         for k in range(Ns):
             t = np.arange(1, R)
             tab[(Ns - 1) + k * (R - 1) + t - 1] = np.exp(-2j * np.pi * ((t * k) % (Ns * R)) / (Ns * R))
         Ns *= R
+    # then the P roots w_P^m of each direct-DFT stage
+    off = L - 1
+    for R in rad:
+        if R not in CODELETS:
+            tab[off:off + R] = np.exp(-2j * np.pi * np.arange(R) / R)
+            off += R
     return tab
+
+
+def _big_stage(lds, tw, L, R, Ns, Tl, sh, roots_off, P):
+    """The direct-DFT stage: work item w = jl + h*Tl (h < BIG_HOLD) covers
+    butterfly j = w % Lr and output pairs k = g*BIG_KP+1 .. of group g = w // Lr
+    (plus X_0 for g = 0); reads unstepped, writes base + k*Ns."""
+    Lr = L // R
+    H = (R - 1) // 2
+    G = (H + BIG_KP - 1) // BIG_KP
+    if Tl * BIG_HOLD < Lr * G:
+        raise ScheduleError(f"L={L} direct radix {R}: {Tl} threads x {BIG_HOLD} items < {Lr * G}")
+    if roots_off + R > tw.size:
+        raise ScheduleError("direct-DFT roots beyond the stage table")
+    roots = tw[roots_off:roots_off + R]
+    if not np.allclose(roots, np.exp(-2j * np.pi * np.arange(R) / R), rtol=0, atol=1e-15):
+        raise ScheduleError("direct-DFT roots wrong")
+    new = np.full(P, np.nan + 0j)
+    hits = np.zeros(P, dtype=np.int64)
+    t = np.arange(R)
+    w = np.arange(Tl * BIG_HOLD)
+    w = w[w < Lr * G]
+    g, j = w // Lr, w % Lr
+    v = lds[_pad(j[:, None] + t[None, :] * Lr, sh)]
+    k = j % Ns
+    if Ns > 1:
+        widx = (Ns - 1) + k[:, None] * (R - 1) + t[None, 1:] - 1
+        if widx.max() >= L - 1:
+            raise ScheduleError("stage twiddle index out of range")
+        v[:, 1:] = v[:, 1:] * tw[widx]
+    V = np.fft.fft(v, axis=1)
+    base = (j // Ns) * Ns * R + k
+    for row in range(w.size):
+        ks = [kk for kk in range(g[row] * BIG_KP + 1, g[row] * BIG_KP + BIG_KP + 1) if kk <= H]
+        outs = ks + [R - kk for kk in ks] + ([0] if g[row] == 0 else [])
+        for kk in outs:
+            dst = _pad(base[row] + kk * Ns, sh)
+            if dst >= P:
+                raise ScheduleError("stage write out of range")
+            new[dst] = V[row, kk]
+            hits[dst] += 1
+    return new, hits
 
 
 def _pad(i, sh):
@@ -72,8 +124,17 @@ def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None) -> None:
     lds[_pad(np.arange(L), sh)] = x
     pmask = (1 << sh) - 1 if sh else 0
     Ns = 1
+    roots_off = L - 1
     for R in rad:
         Lr = L // R
+        if R not in CODELETS:
+            new, hits = _big_stage(lds, tw, L, R, Ns, Tl, sh, roots_off, P)
+            roots_off += R
+            if not np.all(hits[_pad(np.arange(L), sh)] == 1) or hits.sum() != L:
+                raise ScheduleError(f"L={L} direct radix {R}: slots written {np.unique(hits)} times")
+            lds = new
+            Ns *= R
+            continue
         NB = EPT // R
         if NB < 1:
             raise ScheduleError(f"radix {R} exceeds {EPT} registers")

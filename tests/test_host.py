@@ -247,6 +247,30 @@ def test_schedule_batch_uneven():
         off += n
 
 
+DIRECT_PRIME_LENGTHS = [17, 19, 23 * 16, 31 * 27, 37 * 64, 41 * 43, 47 * 128, 53 * 35, 59 * 3, 61 * 61, 67 * 2,
+                        71 * 256, 71 * 71 * 4, 17 * 4096, 29 * 31 * 37, 43 * 1024, 67 * 67]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("n", DIRECT_PRIME_LENGTHS)
+def test_schedule_direct_primes(n, prec):
+    """Primes 17..71 (below the default BLUESTEIN_CUTOFF of 73) run as direct
+    DFT stages (FFTA's DFT leaf, src/algos.jl:135-160), not as convolutions:
+    every launch is a plain Stockham pass, and the emulator replays the
+    stage's work items, write map and root table against numpy's FFT."""
+    s = P.debug_schedule((n,), prec, -1)
+    assert all(la["load_mode"] == 0 and la["mid_mode"] == 0 and la["store_mode"] == 0 for la in s["launches"]), n
+    assert any(r >= 17 for la in s["launches"] for r in la["rad"]), n
+    _check_schedule((n,), prec, -1)
+    _check_schedule((n,), prec, 1, None, 64 * 1024)
+
+
+@pytest.mark.parametrize("shape,region", [((51, 71), None), ((19, 5, 23), (1, 3)), ((64, 47), 2), ((1000, 59), 2)])
+def test_schedule_direct_primes_nd(shape, region):
+    for prec in (32, 64):
+        _check_schedule(shape, prec, -1, region)
+
+
 @pytest.mark.parametrize("n,prec", [(20011, 32), (3199, 32), (28693, 32), (65521, 32), (10007, 64), (4036, 64),
                                     (3 * 4099, 32)])
 def test_register_passes(n, prec):
