@@ -48,6 +48,10 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, FastDiv f) {
 enum LoadMode : int32_t {
   LOAD_PLAIN = 0,  // v = x
   LOAD_CHIRP = 1,  // Bluestein: v = x[n] * conj(chirp[n]) for n < chirp_n, 0 beyond
+  // Rader: the line holds the chirp_n = P elements of a prime-length DFT;
+  // element t >= 1 goes to LDS slot rtab[t] (a' = x permuted by powers of a
+  // primitive root), element 0 to the line's side slot x0
+  LOAD_RADER = 2,
 };
 
 // What a line pass does between its first and second radix sweep.
@@ -61,7 +65,14 @@ enum StoreMode : int32_t {
   STORE_PLAIN = 0,  // y = v
   STORE_BHAT = 1,   // Bluestein spectrum product: y = conj(v * bhat[n])
   STORE_CHIRP = 2,  // Bluestein end: y = conj(chirp[n] * v) for n < chirp_n only
+  // Rader end: y[0] = x0 + v0 (v0 = the first sweep's DC term, side slot),
+  // y[k] = x0 + conj(v[m]) with m = (L - rtab[k]) mod L, L = chirp_n - 1
+  STORE_RADER = 3,
 };
+
+// LDS side slots of a Rader line: the two complex entries after the line's
+// padded positions (the planner adds them to the pitch).
+constexpr int kRaderSide = 2;
 
 // One batched pass of in-LDS line FFTs.  A "line" is L elements of a
 // transform; every line is independently transformed (forward sign), with
@@ -140,6 +151,9 @@ struct LinePass {
   // exp(i pi (t^2 s^2 mod 2N) / N) from this FP64 table (L entries; null:
   // read the chirp table instead).
   const double2* chirp_d;
+  // Rader passes: LDS slot of each element t < chirp_n of the prime-length
+  // line (rtab[0] unused)
+  const uint16_t* rtab;
 };
 
 // Radix schedule of an in-LDS line (for table generation).

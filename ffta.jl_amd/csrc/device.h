@@ -36,6 +36,7 @@ __device__ __forceinline__ double2 gld(const double2* p) {
   return make_double2(v.x, v.y);
 }
 __device__ __forceinline__ int64_t gld(const int64_t* p) { return *(const FFTAHIP_GLOBAL int64_t*)p; }
+__device__ __forceinline__ int gld(const uint16_t* p) { return *(const FFTAHIP_GLOBAL uint16_t*)p; }
 // A table entry that every lane of a wave reads at the same address, through
 // the constant address space (a scalar load into SGPRs, no VGPRs held).
 __device__ __forceinline__ double2 uld(const double2* p) {
@@ -103,7 +104,7 @@ __device__ __forceinline__ int pad_idx(int i, int she) { return i + (i >> she); 
 template <class Rt, int RAD, int EPT, bool FUSE, class LdF, class StF>
 __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl, int L, int Ns, FastDiv nsd,
                                       const cx<Rt>* __restrict__ tw, int sh, bool gload, bool gstore, const LdF& ld,
-                                      const StF& st, const cx<Rt>* __restrict__ pm, int64_t pms) {
+                                      const StF& st, const cx<Rt>* __restrict__ pm, int pms) {
   constexpr int NB = EPT / RAD;
   const int Lr = L / RAD;
   const int she = pad_shift(sh);
@@ -125,7 +126,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       cx<Rt> f[kPre ? RAD : 1];
       if (kPre) {
 #pragma unroll
-        for (int t = 0; t < RAD; ++t) f[kPre ? t : 0] = gld(pm + (int64_t)(j + t * Lr) * pms);
+        for (int t = 0; t < RAD; ++t) f[kPre ? t : 0] = gld(pm + (j + t * Lr) * pms);
       }
       if (rstep) {
         const int p0 = pad_idx(j, she);
@@ -137,7 +138,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       }
 #pragma unroll
       for (int t = 0; t < RAD; ++t)
-        v[b][t] = cconj(cmul(v[b][t], kPre ? f[kPre ? t : 0] : gld(pm + (int64_t)(j + t * Lr) * pms)));
+        v[b][t] = cconj(cmul(v[b][t], kPre ? f[kPre ? t : 0] : gld(pm + (j + t * Lr) * pms)));
       dft_any<Rt, RAD>(v[b]);
     } else if (active && j < Lr) {
       // FP32 pre-twiddle and full passes: the stage twiddles are requested
@@ -209,7 +210,8 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
 // A work item is (butterfly j, group g of kBigKP consecutive k); butterflies
 // run fastest across threads, and a thread holds at most kBigHold work
 // items across the barrier (the planner sizes Tl for that).  `roots` holds
-// w_P^m = exp(-2 pi i m/P), m < P, in the pass precision; the exponent j*k
+// w_P^m = exp(-2 pi i m/P), m < P, in the pass precision, staged in LDS
+// (the tile copies them there before its first barrier); the exponent j*k
 // mod P advances by k per step.  Stage twiddles (Ns > 1) are applied to the
 // inputs as they are read, from the same per-stage table as stage().
 constexpr int kBigKP = 4;
@@ -247,6 +249,9 @@ __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int
     }
     cx<Rt> sum = x0;
     const int k0 = g * kBigKP + 1;
+    // two steps in flight for FP32; FP64 has no registers for a second
+    constexpr int kUnroll = sizeof(Rt) == 4 ? 2 : 1;
+#pragma unroll kUnroll
     for (int i = 1; i <= H; ++i) {
       cx<Rt> a = line[pad_idx(j + i * Lr, she)];
       cx<Rt> b = line[pad_idx(j + (P - i) * Lr, she)];
@@ -260,7 +265,7 @@ __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int
       for (int u = 0; u < kBigKP; ++u) {
         m[u] += k0 + u;
         if (m[u] >= P) m[u] -= P;
-        const cx<Rt> r = gld(roots + m[u]);  // (cos, -sin) of 2 pi m / P
+        const cx<Rt> r = roots[m[u]];  // (cos, -sin) of 2 pi m / P
         A[u].x += r.x * s.x;
         A[u].y += r.x * s.y;
         B[u].x -= r.y * d.x;
@@ -298,13 +303,14 @@ __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int
 // fuse_last the last stage writes through st(element, value).
 template <class Rt, int EPT, bool FUSE, bool BIG, class LdF, class StF>
 __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const LinePass& p,
-                                      const cx<Rt>* __restrict__ tw, bool fuse_first, bool fuse_last, const LdF& ld,
-                                      const StF& st, const cx<Rt>* __restrict__ pm = nullptr, int64_t pms = 0) {
+                                      const cx<Rt>* __restrict__ tw, const cx<Rt>* roots, bool fuse_first,
+                                      bool fuse_last, const LdF& ld, const StF& st,
+                                      const cx<Rt>* __restrict__ pm = nullptr, int pms = 0) {
   int Ns = 1;
   const int sh = p.lds_sh;
   const int nst = p.nst;
-  // roots of the radix-P stages with P >= 17 follow the L - 1 stage entries
-  int roots_off = p.L - 1;
+  // roots of the radix-P stages with P >= 17, one block of P per stage (LDS)
+  int roots_off = 0;
   for (int s = 0; s < nst; ++s) {
     const int r = p.rad[s];
     const FastDiv nsd = p.nsdiv[s];
@@ -313,7 +319,7 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
     if (BIG && r >= 17) {
       // the planner keeps fused global access and the in-stage spectrum
       // product off stages of this kind
-      stage_big<Rt>(line, jl, active, p.Tl, p.L, r, Ns, nsd, tw, tw + roots_off, sh);
+      stage_big<Rt>(line, jl, active, p.Tl, p.L, r, Ns, nsd, tw, roots + roots_off, sh);
       roots_off += r;
       Ns *= r;
       continue;
@@ -339,6 +345,16 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
 }
 
 // ------------------------------------------------------------------ line pass
+
+// Positions t < live_bound(rem, step, L) of a line satisfy t*step < rem
+// (rem = chirp_n minus the line's first chirp index): the live prefix of a
+// Bluestein line, as a 32-bit bound computed once per line.
+__device__ __forceinline__ int live_bound(int64_t rem, int64_t step, int L) {
+  if (rem <= 0) return 0;
+  if (step <= 0) return L;
+  const int64_t b = (rem + step - 1) / step;
+  return b < (int64_t)L ? (int)b : L;
+}
 
 // Per-line offsets of a tile (input, output, pass-twiddle index, chirp in /
 // out, spectrum), kept in LDS right after the tile's lines and sized by its
@@ -462,6 +478,17 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const int64_t rem = p.nlines - line0;
   const int nvalid = rem < C ? (int)rem : C;
 
+  // direct-DFT roots (BIG variants): the stage table's entries after its
+  // L - 1 stage twiddles, staged after the line tables
+  cx<Rt>* roots = nullptr;
+  if (BIG) {
+    roots = reinterpret_cast<cx<Rt>*>(reinterpret_cast<unsigned char*>(lds) + tile_lds_bytes(C, ldl, sizeof(cx<Rt>)));
+    int nroots = 0;
+    for (int s = 0; s < p.nst; ++s)
+      if (p.rad[s] >= 17) nroots += p.rad[s];
+    const cx<Rt>* __restrict__ src = reinterpret_cast<const cx<Rt>*>(p.tw) + (L - 1);
+    for (int i = tid; i < nroots; i += BLOCK) roots[i] = gld(src + i);
+  }
   for (int c = tid; c < nvalid; c += BLOCK) {
     const LineOffsets o = line_offsets<MODE>(p, (uint32_t)(line0 + c));
     sh.in[c] = o.in;
@@ -489,6 +516,9 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const bool pretw_in = MODE != MODE_PLAIN && p.pretw == 1;
   const bool pretw_out = MODE == MODE_FULL && p.pretw == 2;
   const int store_mode = MODE == MODE_FULL ? p.store_mode : STORE_PLAIN;
+  // Rader line: P = chirp_n elements on the global side, L = P - 1 in LDS,
+  // x0 and the first sweep's DC term in the two side slots after the line
+  const bool rader = MODE == MODE_FULL && p.load_mode == LOAD_RADER;
 
   const bool fuse_in = MODE == MODE_PLAIN && p.fuse_in != 0;
   const bool fuse_out = MODE == MODE_PLAIN && p.fuse_out != 0;
@@ -514,13 +544,51 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   // ---- load (global -> LDS), coalesced across lines or along a line.  All
   // global loads of a batch (element, chirp factor, pass-twiddle halves) are
   // issued before any of them is consumed.
-  if (!fuse_in && (p.col_in || p.row_in) && col_step) {
+  if (rader) {
+    // element t >= 1 of the prime-length line to slot rtab[t], element 0 to
+    // the x0 side slot (the planner gives Rader passes a stepped layout)
+    const int in_c = p.col_in ? my_c : row_c;
+    const int in_t0 = p.col_in ? my_t0 : row_t0;
+    if (in_c < nvalid) {
+      const cx<Rt>* __restrict__ src = in + sh.in[in_c];
+      cx<Rt>* line = lds + in_c * ldl;
+      const int P = L + 1;
+      for (int t0 = in_t0; t0 < P; t0 += tstep * U) {
+        cx<Rt> v[U];
+        int slot[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * tstep;
+          slot[u] = -1;
+          if (t < P) {
+            v[u] = gld(src + (int64_t)t * p.ae);
+            slot[u] = t ? pad_idx(gld(p.rtab + t), pshe) : ldl - kRaderSide;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (slot[u] >= 0) line[slot[u]] = cconj_if(v[u], sgn_in);
+      }
+    }
+  } else if (!fuse_in && (p.col_in || p.row_in) && col_step) {
     const int in_c = p.col_in ? my_c : row_c;
     const int in_t0 = p.col_in ? my_t0 : row_t0;
     if (in_c < nvalid) {
       const int c = in_c;
-      const cx<Rt>* __restrict__ src = in + sh.in[c];
-      const int64_t cin0 = MODE == MODE_FULL ? sh.cin[c] : 0;
+      // the element and chirp pointers advance by a fixed stride per step
+      // (no per-element 64-bit products); positions t < tlive have a chirp
+      // factor (n = cin0 + t*cae < chirp_n), the others load zero
+      const int64_t sstep = (int64_t)tstep * p.ae;
+      const cx<Rt>* __restrict__ sp = in + sh.in[c] + (int64_t)in_t0 * p.ae;
+      const cx<Rt>* __restrict__ cp = chirp;
+      int64_t cstep = 0;
+      int tlive = L;
+      if (chirp_in) {
+        const int64_t cin0 = sh.cin[c];
+        cp = chirp + cin0 + (int64_t)in_t0 * p.cae;
+        cstep = (int64_t)tstep * p.cae;
+        tlive = live_bound(p.chirp_n - cin0, p.cae, L);
+      }
       double2 w = make_double2(1.0, 0.0), wr = w;
       // exponents k*t*s with t < L only (the tables end at M)
       if (kGeo && pretw_in && in_t0 < L) {
@@ -535,16 +603,13 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
           const int t = t0 + u * tstep;
           v[u] = {(Rt)0, (Rt)0};
           f[u] = {(Rt)0, (Rt)0};
-          if (t < L) {
-            bool live = true;
-            if (chirp_in) {
-              const int64_t n = cin0 + (int64_t)t * p.cae;
-              live = n < p.chirp_n;
-              if (live) f[u] = gld(chirp + n);
-            }
-            if (live) v[u] = gld(src + (int64_t)t * p.ae);
-            if (!kGeo && pretw_in) pw[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
+          if (t < tlive) {
+            if (chirp_in) f[u] = gld(cp);
+            v[u] = gld(sp);
           }
+          if (t < L && !kGeo && pretw_in) pw[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
+          sp += sstep;
+          cp += cstep;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -650,15 +715,22 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   };
   // only the plain variant carries the fused global paths (register cost)
   constexpr bool kFuse = MODE == MODE_PLAIN;
-  sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, fuse_in, fuse_out && !has_mid, ld, st);
+  sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, fuse_in, fuse_out && !has_mid, ld, st);
+  if (rader) {
+    // the DC term of the first sweep (sum of elements 1..P-1) before the
+    // spectrum product overwrites it
+    if (active && jl == 0) myline[ldl - kRaderSide + 1] = myline[0];
+    __syncthreads();
+  }
 
   if (has_mid && p.mid_in_stage) {
-    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, false, fuse_out, ld, st, bhat + sh.cmid[cs], p.cme);
+    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, bhat + sh.cmid[cs], (int)p.cme);
   } else if (has_mid) {
     if (p.row_mid && col_step) {
       if (row_c < nvalid) {
         const int c = row_c;
-        const cx<Rt>* __restrict__ fsrc = bhat + sh.cmid[c];
+        const cx<Rt>* __restrict__ fp = bhat + sh.cmid[c] + (int64_t)row_t0 * p.cme;
+        const int64_t fstep = (int64_t)tstep * p.cme;
         cx<Rt>* line = lds + c * ldl;
         for (int t0 = row_t0; t0 < L; t0 += tstep * U) {
           cx<Rt> f[U], x[U];
@@ -666,9 +738,10 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
           for (int u = 0; u < U; ++u) {
             const int t = t0 + u * tstep;
             if (t < L) {
-              f[u] = gld(fsrc + (int64_t)t * p.cme);
+              f[u] = gld(fp);
               x[u] = line[pad_idx(t, pshe)];
             }
+            fp += fstep;
           }
 #pragma unroll
           for (int u = 0; u < U; ++u) {
@@ -699,7 +772,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         if (slot[u] >= 0) lds[slot[u]] = cconj(cmul(lds[slot[u]], f[u]));
     }
     __syncthreads();
-    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, false, fuse_out, ld, st);
+    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st);
   }
 
   // ---- store (LDS -> global).  As on the load side, the table factors of a
@@ -743,13 +816,50 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     }
     return;
   }
+  if (rader) {
+    // X_0 = x0 + v0; X_k = x0 + conj(v[m]), m = (L - rtab[k]) mod L
+    const int oc = p.col_out ? my_c : row_c;
+    const int ot0 = p.col_out ? my_t0 : row_t0;
+    if (oc < nvalid) {
+      cx<Rt>* __restrict__ dst = out + sh.out[oc];
+      const cx<Rt>* line = lds + oc * ldl;
+      const cx<Rt> x0 = line[ldl - kRaderSide];
+      const cx<Rt> v0 = line[ldl - kRaderSide + 1];
+      const int P = L + 1;
+      for (int t0 = ot0; t0 < P; t0 += tstep * U) {
+        int r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * tstep;
+          r[u] = (t > 0 && t < P) ? gld(p.rtab + t) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u * tstep;
+          if (t >= P) continue;
+          cx<Rt> y = t == 0 ? cadd(x0, v0) : cadd(x0, cconj(line[pad_idx(r[u] ? L - r[u] : 0, pshe)]));
+          y = cconj_if(y, sgn_out);
+          if (has_scale) y = cscale(y, scale);
+          gst(dst + (int64_t)t * p.be, y);
+        }
+      }
+    }
+    return;
+  }
   if ((p.col_out || p.row_out) && col_step) {
     const int oc = p.col_out ? my_c : row_c;
     const int ot0 = p.col_out ? my_t0 : row_t0;
     if (oc < nvalid) {
       const int c = oc;
-      cx<Rt>* __restrict__ dst = out + sh.out[c];
+      // output and table pointers advance by a fixed stride per step; a
+      // chirp store writes positions t < tlive only (n < chirp_n)
+      const int64_t dstep = (int64_t)tstep * p.be;
+      cx<Rt>* __restrict__ dp = out + sh.out[c] + (int64_t)ot0 * p.be;
+      const cx<Rt>* __restrict__ tab = store_mode == STORE_BHAT ? bhat : chirp;
       const int64_t cout0 = sh.cout[c];
+      const cx<Rt>* __restrict__ fp = tab + cout0 + (int64_t)ot0 * p.cbe;
+      const int64_t fstep = (int64_t)tstep * p.cbe;
+      const int tlive = store_mode == STORE_CHIRP ? live_bound(p.chirp_n - cout0, p.cbe, L) : L;
       double2 w = make_double2(1.0, 0.0), wr = w;
       // exponents k*t*s with t < L only (the tables end at M)
       if (kGeo && pretw_out && ot0 < L) {
@@ -759,42 +869,37 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
       for (int t0 = ot0; t0 < L; t0 += tstep * U) {
         cx<Rt> x[U], f[U];
         PassTw<Rt> pw[U];
-        bool ok[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int t = t0 + u * tstep;
-          ok[u] = t < L;
-          if (!ok[u]) continue;
-          if (store_mode != STORE_PLAIN) {
-            const int64_t n = cout0 + (int64_t)t * p.cbe;
-            if (store_mode == STORE_BHAT) {
-              f[u] = gld(bhat + n);
-            } else {
-              ok[u] = n < p.chirp_n;
-              f[u] = gld(chirp + (ok[u] ? n : 0));
-            }
+          x[u] = f[u] = {(Rt)0, (Rt)0};
+          if (t < tlive) {
+            if (store_mode != STORE_PLAIN) f[u] = gld(fp);
+            if (!kGeo && pretw_out) pw[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
+            x[u] = lds[c * ldl + pad_idx(t, pshe)];
           }
-          if (!kGeo && pretw_out) pw[u] = pass_twiddle_load<Rt>(p, sh.twk[c], t);
-          x[u] = lds[c * ldl + pad_idx(t, pshe)];
+          fp += fstep;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int t = t0 + u * tstep;
-          if (t >= L) continue;
-          cx<Rt> y = x[u];
-          if (pretw_out) {
-            if (kGeo) {
-              y = cmul(y, from_d2<Rt>(w));
-              w = dmul2(w, wr);
-            } else {
-              y = cmul(y, pass_twiddle_value<Rt>(pw[u]));
+          if (t < L) {
+            cx<Rt> y = x[u];
+            if (pretw_out) {
+              if (kGeo) {
+                y = cmul(y, from_d2<Rt>(w));
+                w = dmul2(w, wr);
+              } else {
+                y = cmul(y, pass_twiddle_value<Rt>(pw[u]));
+              }
             }
+            if (store_mode == STORE_BHAT) y = cconj(cmul(y, f[u]));
+            else if (store_mode == STORE_CHIRP) y = cconj(cmul(f[u], y));
+            y = cconj_if(y, sgn_out);
+            if (has_scale) y = cscale(y, scale);
+            if (t < tlive) gst(dp, y);
           }
-          if (store_mode == STORE_BHAT) y = cconj(cmul(y, f[u]));
-          else if (store_mode == STORE_CHIRP) y = cconj(cmul(f[u], y));
-          y = cconj_if(y, sgn_out);
-          if (has_scale) y = cscale(y, scale);
-          if (ok[u]) gst(dst + (int64_t)t * p.be, y);
+          dp += dstep;
         }
       }
     }

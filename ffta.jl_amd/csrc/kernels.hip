@@ -245,6 +245,20 @@ __global__ void gen_chirp_steps_kernel(double2* d, int64_t s, int64_t N, int L) 
   d[t] = chirp_root(u * u, N);
 }
 
+// Rader's kernel b_r = w_P^{g^r}, r < P - 1 (g a primitive root of P), FP64;
+// g^r mod P by binary powering (P < 2^31, products below 2^62).
+__global__ void gen_rader_b_kernel(double2* b, int64_t P, int64_t g) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P - 1) return;
+  uint64_t acc = 1, base = (uint64_t)g, e = (uint64_t)r;
+  while (e) {
+    if (e & 1) acc = acc * base % (uint64_t)P;
+    base = base * base % (uint64_t)P;
+    e >>= 1;
+  }
+  b[r] = unit_root((int64_t)acc, P);
+}
+
 // Periodic extension of the chirp into M slots (b[m], b[M-m] = b[m] for
 // 1 <= m < N, zeros between), FP64.
 __global__ void gen_chirp_ext_kernel(double2* bext, const double2* chirp64, int64_t N, int64_t M) {
@@ -443,6 +457,11 @@ hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_
 
 hipError_t launch_gen_chirp_steps(double2* d, int64_t s, int64_t N, int L, hipStream_t stream) {
   hipLaunchKernelGGL(gen_chirp_steps_kernel, dim3((uint32_t)((L + 63) / 64)), dim3(64), 0, stream, d, s, N, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_rader_b(double2* b, int64_t P, int64_t g, hipStream_t stream) {
+  hipLaunchKernelGGL(gen_rader_b_kernel, dim3((uint32_t)((P - 1 + 255) / 256)), dim3(256), 0, stream, b, P, g);
   return hipGetLastError();
 }
 

@@ -86,7 +86,7 @@ static DevInfo& dev_info() {
 static thread_local bool g_dry = false;
 static thread_local int64_t g_dry_lds = 0;
 enum TokenKind { TK_STAGE = 1, TK_TWLO = 2, TK_TWHI = 3, TK_CHIRP = 4, TK_BHAT = 5, TK_IN = 6, TK_OUT = 7,
-                 TK_SCRATCH = 8, TK_BHATROWS = 9, TK_OFFTAB = 10, TK_TW16 = 11, TK_CHIRPD = 12 };
+                 TK_SCRATCH = 8, TK_BHATROWS = 9, TK_OFFTAB = 10, TK_TW16 = 11, TK_CHIRPD = 12, TK_RTAB = 13 };
 static void* token(int kind, int64_t v) {
   return (void*)(uintptr_t)((1ull << 60) | ((uint64_t)kind << 52) | (uint64_t)v);
 }
@@ -224,8 +224,11 @@ static bool codelet_radix(int r) {
 }
 
 // Largest prime the direct-DFT stage takes (larger primes go through a
-// convolution whatever BLUESTEIN_CUTOFF says).
-constexpr int64_t kMaxDirectPrime = 1021;
+// convolution whatever BLUESTEIN_CUTOFF says), and the most roots one
+// schedule's direct-DFT stages may stage in LDS (their sum of primes; 760
+// FP64 entries fit the LDS headroom kept beside the largest line).
+constexpr int64_t kMaxDirectPrime = 751;
+constexpr int64_t kMaxDirectRoots = 760;
 // Direct-DFT stage geometry (device.h): work items of kBigKP output pairs,
 // kBigHold per thread.
 constexpr int64_t kBigKPHost = 4, kBigHoldHost = 2;
@@ -248,7 +251,10 @@ static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in) {
     }
     prod *= r;
   }
-  if (prod != L || Tl > 1024) return false;
+  int64_t roots = 0;
+  for (int r : rad)
+    if (!codelet_radix(r)) roots += r;
+  if (prod != L || Tl > 1024 || roots > kMaxDirectRoots) return false;
   in = Inner();
   in.L = L;
   in.rad = rad;
@@ -385,7 +391,8 @@ static bool smooth13(int64_t n) {
 
 // ------------------------------------------------------------------ length plan
 
-enum Kind { K_SINGLE = 0, K_MULTI = 1, K_BLUE_SINGLE = 2, K_BLUE_MULTI = 3, K_BLUE_3PASS = 4, K_MIXED = 5 };
+enum Kind { K_SINGLE = 0, K_MULTI = 1, K_BLUE_SINGLE = 2, K_BLUE_MULTI = 3, K_BLUE_3PASS = 4, K_MIXED = 5,
+            K_RADER_SINGLE = 6, K_RADER_MIXED = 7 };
 static const char* kind_name(int k) {
   switch (k) {
     case K_SINGLE: return "stockham-1pass";
@@ -393,6 +400,8 @@ static const char* kind_name(int k) {
     case K_BLUE_SINGLE: return "bluestein-1kernel";
     case K_BLUE_3PASS: return "bluestein-3pass";
     case K_MIXED: return "bluestein-line+stockham";
+    case K_RADER_SINGLE: return "rader-1kernel";
+    case K_RADER_MIXED: return "rader-line+stockham";
     default: return "bluestein-multipass";
   }
 }
@@ -412,11 +421,17 @@ struct LengthPlan {
   int64_t cutoff = 73;
   std::vector<FftaNode> ffta;
   bool blue = false;  // N has a prime factor > 13
-  bool mixed = false; // N = S*Q: Q through an in-LDS Bluestein line, S through Stockham passes
+  bool mixed = false; // N = S*Q: Q through an in-LDS Bluestein (or Rader) line, S through Stockham passes
+  // the convolution of the prime bq runs Rader's algorithm: a cyclic
+  // convolution of M = bq - 1 points in one workgroup (instead of Bluestein's
+  // M >= 2 bq - 1)
+  bool rader = false;
+  int64_t rg = 0;     // primitive root of bq (Rader)
   int64_t S = 1;      // 13-smooth part of N
   int64_t bq = 1;     // length that goes through Bluestein (N, or Q when mixed)
   int64_t M = 1;      // padded Bluestein length (>= 2*bq-1), or N
-  DevBuf chirp, bhat, bhat_rows;
+  DevBuf chirp, bhat, bhat_rows, rtab;
+  const uint16_t* rtab_ptr = nullptr;  // Rader slot of each element (rtab.p, or a token in dry-run mode)
   const void* chirp_ptr = nullptr;  // chirp.p, or a token in dry-run mode
   const void* bhat_ptr = nullptr;
   const void* bhat_rows_ptr = nullptr;  // spectrum permuted for the fused row pass (M = M1*M2)
@@ -613,8 +628,9 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
   d = Decomp();
   const int prec = lp.prec;
   if (lp.mixed) {
-    // pass 0: Q-point DFTs as Bluestein lines of M in LDS; then Stockham over S
-    d.kind = K_MIXED;
+    // pass 0: Q-point DFTs as Bluestein lines of M in LDS (or Rader lines of
+    // Q - 1); then Stockham over S
+    d.kind = lp.rader ? K_RADER_MIXED : K_MIXED;
     d.M = lp.N;
     std::vector<int64_t> FS;
     Inner tmp;
@@ -637,6 +653,16 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
   // the user length's own primes follow the plan's cutoff; Bluestein's M is internal
   const int64_t cut = lp.blue ? kInternalCutoff : lp.cutoff;
   d.M = target;
+  if (lp.rader) {
+    // one workgroup per line whatever the layout (a strided line that does
+    // not fit pass_max runs with fewer neighbouring lines per tile)
+    Inner in;
+    if (!make_inner(target, in, prec, cut)) return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no line schedule for Rader");
+    d.kind = K_RADER_SINGLE;
+    d.F = {target};
+    d.inner = {in};
+    return FFTAHIP_SUCCESS;
+  }
   const int64_t single_cap = strided ? pass_max(prec) : single_max(prec);
   if (target <= single_cap) {
     Inner in;
@@ -870,9 +896,15 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
     const int64_t blk = thr <= 256 ? 256 : (thr <= 512 ? 512 : 1024);
     C = std::max<int64_t>(C, std::min<int64_t>(kMaxLines, blk / in.Tl));
   }
-  const int64_t padmax = in.L + (in.L - 1) / 16 + 1;  // largest pitch any padding choice needs
+  // Rader lines keep x0 and the first sweep's DC term after their positions
+  const int64_t side = p.load_mode == LOAD_RADER ? kRaderSide : 0;
+  const int64_t padmax = in.L + (in.L - 1) / 16 + 1 + side;  // largest pitch any padding choice needs
   const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pass_budget(), padmax * esz)) : lds_budget();
-  while (C > 1 && C * padmax * esz > budget) --C;
+  // direct-DFT stages stage their roots after the tile's line tables
+  int64_t roots_bytes = 0;
+  for (int r : in.rad)
+    if (!codelet_radix(r)) roots_bytes += r * esz;
+  while (C > 1 && C * padmax * esz + roots_bytes > budget) --C;
   if (C > p.nlines) C = std::max<int64_t>(1, p.nlines);
   // LDS padding: env FFTAHIP_LDS_PAD (-1 = choose by the bank model)
   const int64_t force = env_i64("FFTAHIP_LDS_PAD", -1);
@@ -882,7 +914,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
   const int words = (int)(esz / 4);
   static std::mutex pad_mu;
   static std::map<std::vector<int64_t>, std::pair<int, int64_t>> pad_cache;
-  std::vector<int64_t> key = {in.L, in.Tl, C, words, p.col_in, p.col_out, force};
+  std::vector<int64_t> key = {in.L, in.Tl, C, words, p.col_in, p.col_out, force, side};
   for (int r : in.rad) key.push_back(r);
   {
     std::lock_guard<std::mutex> g(pad_mu);
@@ -898,7 +930,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
     if (force >= 0 && sh != force) continue;
     for (int extra : {0, 1}) {
       if (C == 1 && extra) continue;
-      const int64_t pitch = padded((int)in.L - 1, sh) + 1 + extra;
+      const int64_t pitch = padded((int)in.L - 1, sh) + 1 + extra + side;
       const double cst = lds_cost(in, (int)C, (int)pitch, sh, words, p.col_in, p.col_out);
       if (cst < best - 1e-9) {
         best = cst;
@@ -909,7 +941,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
   }
   if (force > 0 && best > 1e29) {
     best_sh = (int)force;
-    ldl = padded((int)in.L - 1, best_sh) + 1;
+    ldl = padded((int)in.L - 1, best_sh) + 1 + side;
   }
   {
     std::lock_guard<std::mutex> g(pad_mu);
@@ -941,9 +973,14 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
     p.row_in = (wide && !p.col_in && !p.fuse_in) ? 1 : 0;
     p.row_out = (wide && !p.col_out && !p.fuse_out) ? 1 : 0;
     p.row_mid = wide ? 1 : 0;
+    // Rader passes load and store through their stepped paths only
+    if (side) {
+      p.row_in = p.col_in ? 0 : 1;
+      p.row_out = p.col_out ? 0 : 1;
+    }
     p.mid_in_stage = (p.mid_mode == MID_BHAT && env_i64("FFTAHIP_MID_STAGE", 1) != 0 && !big_first) ? 1 : 0;
   }
-  L.lds = tile_lds_bytes(C, ldl, esz);
+  L.lds = tile_lds_bytes(C, ldl, esz) + (size_t)roots_bytes;
   L.grid = (p.nlines + C - 1) / C;
   p.xcd_remap = (L.grid >= 16 && L.grid % 8 == 0) ? 1 : 0;
   // short lines laid across memory on both sides (>= 128 contiguous bytes
@@ -1030,6 +1067,21 @@ static void set_out(LinePass& p, BufRef r) {
   p.out_off = r.off;
 }
 
+// Operations of a Rader line pass over the prime lp.bq: load through the
+// slot table, spectrum product between the sweeps, x0 + conj on store; the
+// spectrum is indexed by the position alone (one table for every line).
+static void set_rader(LinePass& p, const LengthPlan& lp) {
+  p.load_mode = LOAD_RADER;
+  p.mid_mode = MID_BHAT;
+  p.store_mode = STORE_RADER;
+  p.chirp_n = lp.bq;
+  p.cae = p.cbe = p.cme = 1;
+  for (int u = 0; u < 4; ++u) p.ca[u] = p.cb[u] = p.cm[u] = 0;
+  p.bhat = lp.bhat_ptr;
+  p.rtab = lp.rtab_ptr;
+  p.chirp = nullptr;
+}
+
 static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& ls, BufRef in, BufRef out, BufRef s0,
                                      BufRef s1, bool conj, std::vector<Launch>& L, hipStream_t stream) {
   const int strided = (ls.es_in != 1 || ls.es_out != 1) ? 1 : 0;
@@ -1041,7 +1093,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
   const int64_t B = ls.count();
   const bool tab = ls.in_tab != nullptr;
 
-  if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE) {
+  if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE || d->kind == K_RADER_SINGLE) {
     Launch la;
     LinePass p = blank_pass();
     set_in(p, in);
@@ -1073,6 +1125,8 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       p.cme = 1;
       p.chirp = lp.chirp_ptr;
       p.bhat = lp.bhat_ptr;
+    } else if (d->kind == K_RADER_SINGLE) {
+      set_rader(p, lp);
     }
     set_inner(p, d->inner[0], prec, la, false);
     la.lp = p;
@@ -1235,7 +1289,11 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       if (i == np - 1) p.store_mode = STORE_BHAT;
       if (last) p.store_mode = STORE_CHIRP;
     }
-    if (d->kind == K_MIXED && s == 0) {
+    if (d->kind == K_RADER_MIXED && s == 0) {
+      // the R = Q point DFT of every line by Rader's algorithm: a line of
+      // Q - 1 points in LDS
+      set_rader(p, lp);
+    } else if (d->kind == K_MIXED && s == 0) {
       // the R = Q point DFT of every line as a Bluestein line of lp.M points
       // in LDS: positions t < Q of the line are loaded and stored
       p.chirp_n = lp.bq;
@@ -1265,8 +1323,10 @@ static fftahip_status scratch_per_line(LengthPlan& lp, int strided, int64_t* ele
   fftahip_status st = get_decomp(lp, strided, &d);
   if (st) return st;
   *elems = d->M;
-  if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE) *nbuf = 0;
-  else if (((d->kind == K_MULTI || d->kind == K_MIXED) && d->F.size() == 2) || d->kind == K_BLUE_3PASS) *nbuf = 1;
+  if (d->kind == K_SINGLE || d->kind == K_BLUE_SINGLE || d->kind == K_RADER_SINGLE) *nbuf = 0;
+  else if (((d->kind == K_MULTI || d->kind == K_MIXED || d->kind == K_RADER_MIXED) && d->F.size() == 2) ||
+           d->kind == K_BLUE_3PASS)
+    *nbuf = 1;
   else *nbuf = 2;
   return FFTAHIP_SUCCESS;
 }
@@ -1311,7 +1371,92 @@ static fftahip_status run_schedule(const DevSchedule& S, int prec, const BufPtrs
   return FFTAHIP_SUCCESS;
 }
 
-// ------------------------------------------------------------------ Bluestein spectrum
+// ------------------------------------------------------------------ Bluestein and Rader spectra
+
+// out = FFT_n(in), forward, in FP64 with a schedule of this library.
+static fftahip_status fft64_device(const void* in, void* out, int64_t n, hipStream_t s) {
+  LengthPlan sub;
+  sub.prec = 64;
+  sub.N = n;
+  sub.M = n;
+  sub.blue = false;
+  Tables T64;
+  T64.prec = 64;
+  int64_t per;
+  int nbuf;
+  fftahip_status st;
+  if ((st = scratch_per_line(sub, 0, &per, &nbuf))) return st;
+  DevBuf s0;
+  if (nbuf >= 1 && (st = s0.ensure((size_t)per * nbuf * 16))) return st;
+  LineSet ls;
+  DevSchedule S;
+  if ((st = build_launches(sub, T64, ls, BufRef{BUF_IN, 0}, BufRef{BUF_OUT, 0}, BufRef{BUF_SCRATCH, 0},
+                           BufRef{BUF_SCRATCH, per}, false, S.L, s)))
+    return st;
+  if ((st = upload_schedule(S))) return st;
+  if ((st = run_schedule(S, 64, BufPtrs{in, out, s0.p}, s))) return st;
+  // the tables and scratch above are freed on return
+  HIP_TRY(hipStreamSynchronize(s));
+  return FFTAHIP_SUCCESS;
+}
+
+static uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+  uint64_t r = 1 % m;
+  b %= m;
+  while (e) {
+    if (e & 1) r = (unsigned __int128)r * b % m;
+    b = (unsigned __int128)b * b % m;
+    e >>= 1;
+  }
+  return r;
+}
+
+// Smallest primitive root of the prime P: g with g^((P-1)/q) != 1 mod P for
+// every prime q dividing P - 1.
+static int64_t primitive_root(int64_t P) {
+  if (P == 2) return 1;
+  std::vector<int64_t> qs = prime_factors(P - 1);
+  qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+  for (int64_t g = 2; g < P; ++g) {
+    bool ok = true;
+    for (int64_t q : qs) ok = ok && powmod((uint64_t)g, (uint64_t)((P - 1) / q), (uint64_t)P) != 1;
+    if (ok) return g;
+  }
+  return 0;
+}
+
+// Rader's algorithm for the prime P = lp.bq (C. M. Rader, 1968): with g a
+// primitive root, a'_q = x_{g^-q} and b_r = w_P^{g^r} (q, r < P - 1),
+//   X_0 = x_0 + sum_q a'_q,   X_{g^m} = x_0 + (a' (*) b)_m,
+// a cyclic convolution of P - 1 points.  The line pass computes it as
+// conj(FFT(conj(FFT(a') * B))) with B = FFT(b) / (P - 1), generated here in
+// FP64 and rounded once; rtab[t] = (-log_g t) mod (P - 1) places element t.
+static fftahip_status prepare_rader(LengthPlan& lp, hipStream_t s) {
+  const int64_t P = lp.bq, L = P - 1;
+  lp.rg = primitive_root(P);
+  if (lp.rg <= 0 || L > 65535) return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no Rader table for " + std::to_string(P));
+  std::vector<uint16_t> tab((size_t)P, 0);
+  uint64_t cur = 1;
+  for (int64_t e = 0; e < L; ++e) {
+    tab[cur] = (uint16_t)((L - e) % L);
+    cur = cur * (uint64_t)lp.rg % (uint64_t)P;
+  }
+  fftahip_status st;
+  if ((st = lp.rtab.ensure((size_t)P * sizeof(uint16_t)))) return st;
+  HIP_TRY(hipMemcpy(lp.rtab.p, tab.data(), (size_t)P * sizeof(uint16_t), hipMemcpyHostToDevice));
+  if ((st = lp.bhat.ensure((size_t)L * csize(lp.prec)))) return st;
+  DevBuf b64, spec;
+  if ((st = b64.ensure((size_t)L * sizeof(double2)))) return st;
+  if ((st = spec.ensure((size_t)L * sizeof(double2)))) return st;
+  HIP_TRY(launch_gen_rader_b((double2*)b64.p, P, lp.rg, s));
+  if ((st = fft64_device(b64.p, spec.p, L, s))) return st;
+  HIP_TRY(launch_finish_bhat(lp.bhat.p, (const double2*)spec.p, lp.prec, L, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  lp.rtab_ptr = (const uint16_t*)lp.rtab.p;
+  lp.bhat_ptr = lp.bhat.p;
+  lp.blue_ready = true;
+  return FFTAHIP_SUCCESS;
+}
 
 // bhat = FFT_M(b_ext) / M computed with an FP64 plan of the same machinery.
 static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s) {
@@ -1320,30 +1465,13 @@ static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s
   if (st) return st;
   st = lp.bhat.ensure((size_t)M * csize(lp.prec));
   if (st) return st;
-  DevBuf c64, bext, spec, s0;
+  DevBuf c64, bext, spec;
   if ((st = c64.ensure((size_t)N * sizeof(double2)))) return st;
   if ((st = bext.ensure((size_t)M * sizeof(double2)))) return st;
   if ((st = spec.ensure((size_t)M * sizeof(double2)))) return st;
   HIP_TRY(launch_gen_chirp(lp.chirp.p, (double2*)c64.p, lp.prec, N, s));
   HIP_TRY(launch_gen_chirp_ext((double2*)bext.p, (const double2*)c64.p, N, M, s));
-  LengthPlan sub;
-  sub.prec = 64;
-  sub.N = M;
-  sub.M = M;
-  sub.blue = false;
-  Tables T64;
-  T64.prec = 64;
-  int64_t per;
-  int nbuf;
-  if ((st = scratch_per_line(sub, 0, &per, &nbuf))) return st;
-  if (nbuf >= 1 && (st = s0.ensure((size_t)per * nbuf * 16))) return st;
-  LineSet ls;
-  DevSchedule S;
-  if ((st = build_launches(sub, T64, ls, BufRef{BUF_IN, 0}, BufRef{BUF_OUT, 0}, BufRef{BUF_SCRATCH, 0},
-                           BufRef{BUF_SCRATCH, per}, false, S.L, s)))
-    return st;
-  if ((st = upload_schedule(S))) return st;
-  if ((st = run_schedule(S, 64, BufPtrs{bext.p, spec.p, s0.p}, s))) return st;
+  if ((st = fft64_device(bext.p, spec.p, M, s))) return st;
   HIP_TRY(launch_finish_bhat(lp.bhat.p, (const double2*)spec.p, lp.prec, M, s));
   int64_t M1, M2;
   if (!lp.mixed && blue3_split(M, lp.prec, &M1, &M2)) {
@@ -1384,6 +1512,36 @@ static int64_t best_line_length(int64_t n, int prec) {
   return best;
 }
 
+// Work of an in-LDS line FFT in stage units per point: 1 per codelet
+// stage, P/div per direct-DFT stage of a prime P (its O(P) work per point;
+// env FFTAHIP_BIG_DIV, default 8).
+static double inner_work(const Inner& in) {
+  const double div = (double)std::max<int64_t>(1, env_i64("FFTAHIP_BIG_DIV", 8));
+  double w = 0;
+  for (int r : in.rad) w += codelet_radix(r) ? 1.0 : r / div;
+  return w;
+}
+
+// Rader (a line of P - 1 points) for the convolution prime P, when P - 1
+// has an in-LDS schedule, fits one workgroup with its side slots, and its
+// sweeps cost no more than the Bluestein line of blue_M points (0: no
+// Bluestein line fits, so Bluestein would take three passes through HBM).
+// Env FFTAHIP_RADER: 0 turns Rader off, 1 allows direct-DFT stages in
+// P - 1, 2 (default) takes only 13-smooth P - 1.
+static bool rader_pick(int64_t P, int prec, int64_t blue_M) {
+  const int64_t mode = env_i64("FFTAHIP_RADER", 2);
+  if (mode == 0) return false;
+  if (P < 5 || P - 1 > 65535 || !is_prime(P)) return false;
+  const int64_t L = P - 1;
+  if (L + kRaderSide > single_max(prec)) return false;
+  Inner in;
+  if (!make_inner(L, in, prec, mode == 2 ? 17 : kInternalCutoff)) return false;
+  Inner b;
+  if (blue_M > 0 && make_inner(blue_M, b, prec) && (double)L * inner_work(in) > (double)blue_M * inner_work(b))
+    return false;
+  return true;
+}
+
 static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t cutoff) {
   lp.prec = prec;
   lp.N = N;
@@ -1395,6 +1553,8 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
   // any other prime factor goes through a convolution
   lp.blue = !direct_smooth(N, cutoff);
   lp.mixed = false;
+  lp.rader = false;
+  lp.rg = 0;
   lp.S = 1;
   lp.bq = N;
   auto smooth5 = [](int64_t v) {
@@ -1417,7 +1577,13 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
     if (m > 1 && direct_prime(m, cutoff)) { S *= m; rest /= m; }
     if (S > 1) {
       const int64_t M = best_line_length(rest, prec);
-      if (M) {
+      if (rader_pick(rest, prec, M)) {
+        lp.mixed = true;
+        lp.rader = true;
+        lp.S = S;
+        lp.bq = rest;
+        lp.M = rest - 1;
+      } else if (M) {
         lp.mixed = true;
         lp.S = S;
         lp.bq = rest;
@@ -1425,7 +1591,11 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
       }
     }
   }
-  if (lp.blue && !lp.mixed) {
+  if (lp.blue && !lp.mixed && rader_pick(N, prec, best_line_length(N, prec))) {
+    lp.rader = true;
+    lp.bq = N;
+    lp.M = N - 1;
+  } else if (lp.blue && !lp.mixed) {
     // smallest 5-smooth length >= 2N-1 that runs in one kernel or three fused
     // passes (FFTA pads to a power of two, src/algos.jl:345; any length >=
     // 2N-1 gives the same convolution)
@@ -1468,13 +1638,18 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
 
 static fftahip_status prepare_length(LengthPlan& lp, Tables& T, hipStream_t s) {
   if (lp.blue && g_dry) {
-    lp.chirp_ptr = token(TK_CHIRP, lp.bq);
+    if (lp.rader) {
+      lp.rg = primitive_root(lp.bq);
+      lp.rtab_ptr = (const uint16_t*)token(TK_RTAB, lp.bq);
+    } else {
+      lp.chirp_ptr = token(TK_CHIRP, lp.bq);
+      lp.bhat_rows_ptr = token(TK_BHATROWS, lp.M);
+    }
     lp.bhat_ptr = token(TK_BHAT, lp.M);
-    lp.bhat_rows_ptr = token(TK_BHATROWS, lp.M);
     lp.blue_ready = true;
     return FFTAHIP_SUCCESS;
   }
-  if (lp.blue && !lp.blue_ready) return prepare_bluestein(lp, T, s);
+  if (lp.blue && !lp.blue_ready) return lp.rader ? prepare_rader(lp, s) : prepare_bluestein(lp, T, s);
   return FFTAHIP_SUCCESS;
 }
 
@@ -2710,6 +2885,8 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   json_ptr(os, "chirp", p.chirp);
   os << ", ";
   json_ptr(os, "chirp_d", p.chirp_d);
+  os << ", ";
+  json_ptr(os, "rtab", p.rtab);
   os << ", ";
   json_ptr(os, "bhat", p.bhat);
   os << "}";

@@ -15,7 +15,8 @@ from __future__ import annotations
 import numpy as np
 
 TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHATROWS, TK_OFFTAB, TK_TW16, \
-    TK_CHIRPD = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12
+    TK_CHIRPD, TK_RTAB = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13
+LOAD_RADER, STORE_RADER, RADER_SIDE = 2, 3, 2
 EPT = 16
 CODELETS = {2, 3, 4, 5, 7, 8, 9, 11, 13, 16}
 # direct-DFT stage of a prime radix P >= 17 (device.h stage_big): work items
@@ -44,6 +45,25 @@ def bhat_of(n: int, m: int) -> np.ndarray:
     ext[:n] = b
     ext[m - np.arange(1, n)] = b[1:]
     return np.fft.fft(ext) / m
+
+
+def primitive_root(P: int) -> int:
+    qs = {q for q in range(2, P) if (P - 1) % q == 0 and all(q % d for d in range(2, int(q ** 0.5) + 1))}
+    return next(g for g in range(2, P) if all(pow(g, (P - 1) // q, P) != 1 for q in qs))
+
+
+def rader_tables(P: int):
+    """Slot table rtab[t] = (-log_g t) mod (P-1) and spectrum FFT(b)/(P-1),
+    b_r = exp(-2 pi i g^r / P), g the smallest primitive root."""
+    L = P - 1
+    g = primitive_root(P)
+    rtab = np.zeros(P, dtype=np.int64)
+    cur = 1
+    for e in range(L):
+        rtab[cur] = (L - e) % L
+        cur = cur * g % P
+    b = np.exp(-2j * np.pi * np.array([pow(g, r, P) for r in range(L)]) / P)
+    return rtab, np.fft.fft(b) / L
 
 
 def stage_table(L: int, rad) -> np.ndarray:
@@ -246,8 +266,9 @@ class Emulator:
             if C < 1 or C > 256 or C * Tl > p["block"]:
                 raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
             sh = p["lds_sh"]
-            need = L + ((L - 1) >> sh if sh else 0)
-            if p["ldl"] < need or p["lds"] != ((C * p["ldl"] * self.esize + 15) & ~15) + C * LINE_TAB_BYTES:
+            need = L + ((L - 1) >> sh if sh else 0) + (RADER_SIDE if p["load_mode"] == LOAD_RADER else 0)
+            roots = sum(r for r in p["rad"] if r not in CODELETS) * self.esize
+            if p["ldl"] < need or p["lds"] != ((C * p["ldl"] * self.esize + 15) & ~15) + C * LINE_TAB_BYTES + roots:
                 raise ScheduleError("LDS size mismatch")
             if p["lds"] > lds_per_block:
                 raise ScheduleError(f"LDS {p['lds']} exceeds {lds_per_block}")
@@ -282,7 +303,15 @@ class Emulator:
         cin = sum(idx[d] * ca[d] for d in range(4))
         cout = sum(idx[d] * cb[d] for d in range(4))
         cmid = sum(idx[d] * p["cm"][d] for d in range(4))
-        t = np.arange(L, dtype=np.int64)
+        rader = p["load_mode"] == LOAD_RADER
+        if rader:
+            if p["rtab"] is None or p["rtab"][0] != TK_RTAB or p["rtab"][1] != L + 1 or p["chirp_n"] != L + 1:
+                raise ScheduleError("Rader pass without its slot table")
+            if p["store_mode"] != STORE_RADER or p["mid_mode"] != 1 or p["pretw"]:
+                raise ScheduleError("Rader pass modes")
+            if not p.get("reg") and not (p["col_in"] or p["row_in"]) or not (p["col_out"] or p["row_out"]):
+                raise ScheduleError("Rader pass without a stepped layout")
+        t = np.arange(L + 1 if rader else L, dtype=np.int64)
         src, so = self._buf(p["in"])
         dst, do = self._buf(p["out"])
         addr_in = so + oin[:, None] + t[None, :] * p["ae"]
@@ -290,7 +319,10 @@ class Emulator:
         chirp = bhat = None
         if p["chirp"] is not None:
             chirp = chirp_fwd(p["chirp"][1])
-        if p["bhat"] is not None:
+        if rader:
+            if p["bhat"] is None or p["bhat"][1] != L:
+                raise ScheduleError("Rader spectrum table")
+        elif p["bhat"] is not None:
             M = p["bhat"][1]
             bhat = bhat_of(p["chirp_n"], M)
             if p["bhat"][0] == TK_BHATROWS:
@@ -320,6 +352,9 @@ class Emulator:
             n = cin[:, None] + t[None, :] * p["cae"]
             nn = np.where(valid_in, n, 0)
             v = np.where(valid_in, v * np.conj(chirp[nn]), 0)
+        if rader:
+            self._rader(p, v, L, cmid, do, oout, dst)
+            return
         if p["pretw"] == 1:
             M = p["tw_lo"][1]
             m = idx[p["twdim"]][:, None] * t[None, :] * p["tw_scale"]
@@ -362,6 +397,38 @@ class Emulator:
         if np.unique(sel).size != sel.size:
             raise ScheduleError("two lines store to one address")
         dst[sel] = v[valid_out]
+
+
+def _rader_pass(p, v, L, cmid, do, oout, dst):
+    P = L + 1
+    rtab, bhat = rader_tables(P)
+    x0 = v[:, 0]
+    a = np.zeros((v.shape[0], L), dtype=np.complex128)
+    a[:, rtab[1:]] = v[:, 1:]
+    V = np.fft.fft(a, axis=1)
+    v0 = V[:, 0]
+    n = cmid[:, None] + np.arange(L)[None, :] * p["cme"]
+    if n.max() >= bhat.size:
+        raise ScheduleError("Rader spectrum index out of range")
+    W = np.fft.fft(np.conj(V * bhat[n]), axis=1)
+    k = np.arange(1, P)
+    m = (L - rtab[k]) % L
+    X = np.empty((v.shape[0], P), dtype=np.complex128)
+    X[:, 0] = x0 + v0
+    X[:, 1:] = x0[:, None] + np.conj(W[:, m])
+    if p["conj_out"]:
+        X = np.conj(X)
+    X = X * p["scale"]
+    addr_out = do + oout[:, None] + np.arange(P)[None, :] * p["be"]
+    sel = addr_out.ravel()
+    if sel.min() < 0 or sel.max() >= dst.size:
+        raise ScheduleError(f"store out of bounds [{sel.min()}, {sel.max()}] of {dst.size}")
+    if np.unique(sel).size != sel.size:
+        raise ScheduleError("two lines store to one address")
+    dst[sel] = X.ravel()
+
+
+Emulator._rader = staticmethod(_rader_pass)
 
 
 def run_plan_schedule(sched: dict, x: np.ndarray, esize: int, lds_per_block: int) -> np.ndarray:

@@ -183,7 +183,7 @@ CUTOFF_CASES = [
     # (length, cutoff): primes below the cutoff run as direct DFT stages,
     # primes at or above it through a convolution (src/callgraph.jl:95-100)
     (5, 2), (5, 7), (13, 11), (13, 73), (17, 17), (17, 18), (71, 73), (71, 71), (101, 73), (101, 200),
-    (199 * 3, 200), (199 * 3, 199), (19 * 16 * 7, 19), (19 * 16 * 7, 20), (997, 1000), (1009 * 2, 2000),
+    (199 * 3, 200), (199 * 3, 199), (19 * 16 * 7, 19), (19 * 16 * 7, 20), (997, 1000), (1009 * 2, 2000), (743, 800), (743 * 4, 744),
     (73 * 73, 73), (73 * 73, 74), (4096, 2),
 ]
 
@@ -216,7 +216,8 @@ def test_bluestein_cutoff_on_device(torch_cuda, n, cutoff, precision):
     for direction in (-1, 1):
         p = Plan((n,), precision, None, direction, cutoff=cutoff)
         d = p.describe()["dims"][0]
-        direct = all(q <= 3 or q < cutoff for q in _prime_factors(n))
+        # the direct-DFT stage takes primes up to 751
+        direct = all(q <= 3 or (q < cutoff and q <= 751) for q in _prime_factors(n))
         kind = d["device"]["contiguous"]["kind"]
         assert kind.startswith("stockham") == direct, (n, cutoff, kind)
         # every prime factor >= 5 ends as a prime leaf of FFTA's tree
@@ -259,3 +260,26 @@ def test_bluestein_cutoff_batch(torch_cuda, cutoff):
     for n, d, o in zip(b.lengths, dirs, b.offsets):
         ref = oracle.fft(x[o:o + n].astype(np.complex128), int(d), cutoff)
         assert oracle.relerr(y[o:o + n], ref) <= tol(int(n), 32), (int(n), cutoff)
+
+
+RADER_SIZES = [79, 101, 103, 139, 257, 641, 769, 1009, 1153, 3457, 4481, 7681, 8191, 12289, 97 * 8, 101 * 27,
+               257 * 64, 3457 * 9, 12289 * 4, 139 * 35]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("direction", [-1, 1])
+@pytest.mark.parametrize("n", RADER_SIZES)
+def test_rader_vs_oracle(torch_cuda, n, direction, precision):
+    """Lengths whose convolution prime P has a smooth P - 1 run Rader's
+    algorithm on the device (FFTA runs Bluestein there, src/algos.jl:389-424;
+    both compute the same DFT)."""
+    from fftahip.plan import Plan
+
+    dt = np.complex64 if precision == 32 else np.complex128
+    x = _rand(n, dt, seed=n * 3 + (direction > 0))
+    p = Plan((n,), precision, None, direction)
+    kind = p.describe()["dims"][0]["device"]["contiguous"]["kind"]
+    print(f"case rader n={n} dir={direction} prec={precision} kind={kind}", flush=True)
+    y = _gpu_transform(torch_cuda, x, direction)
+    err = oracle.relerr(y, _oracle(x, direction))
+    assert err <= tol(n, precision), f"n={n} {kind}: relerr {err:.3e}"

@@ -271,6 +271,44 @@ def test_schedule_direct_primes_nd(shape, region):
         _check_schedule(shape, prec, -1, region)
 
 
+RADER_LENGTHS = [79, 101, 257, 641, 769, 1009, 1153, 3457, 4481, 7681, 8191, 12289, 103, 139, 283,
+                 97 * 8, 101 * 27, 257 * 64, 3457 * 9, 12289 * 4, 139 * 5 * 7]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("n", RADER_LENGTHS)
+def test_schedule_rader(n, prec):
+    """Convolution primes P whose P - 1 has an in-LDS schedule run Rader's
+    algorithm (a cyclic convolution of P - 1 points in one workgroup) instead
+    of Bluestein's (>= 2P - 1 points), on their own or as the prime part of a
+    mixed length; the emulator recomputes the slot table and spectrum from
+    the smallest primitive root and checks the result against numpy's FFT."""
+    s = P.debug_schedule((n,), prec, -1)
+    rad = [la for la in s["launches"] if la["load_mode"] == 2]
+    big = max(q for q in range(2, n + 1) if n % q == 0 and all(q % d for d in range(2, int(q ** 0.5) + 1)))
+    # a Rader line of P - 1 points plus two side slots must fit one workgroup's
+    # LDS; with a direct-DFT stage in P - 1 the cost model may keep Bluestein
+    smooth13 = all(q <= 13 for q in range(2, big) if (big - 1) % q == 0 and all(q % d for d in range(2, q)))
+    if big - 1 <= (16384 if prec == 32 else 8000) and smooth13:
+        assert rad and rad[0]["L"] == big - 1, [(la["L"], la["load_mode"]) for la in s["launches"]]
+    _check_schedule((n,), prec, -1)
+    _check_schedule((n,), prec, 1, None, 64 * 1024)
+
+
+def test_rader_not_for_rough_p_minus_1():
+    """P - 1 with a prime factor above 71 (4099: 4098 = 2 3 683) stays Bluestein."""
+    s = P.debug_schedule((4099,), 32, -1)
+    assert all(la["load_mode"] != 2 for la in s["launches"])
+    assert any(la["load_mode"] == 1 for la in s["launches"])
+
+
+@pytest.mark.parametrize("shape,region", [((101, 12), 1), ((12, 101), 2), ((97 * 8, 3), None), ((5, 257, 2), 2)])
+def test_schedule_rader_nd(shape, region):
+    for prec in (32, 64):
+        _check_schedule(shape, prec, -1, region)
+        _check_schedule(shape, prec, 1, region)
+
+
 @pytest.mark.parametrize("n,prec", [(20011, 32), (3199, 32), (28693, 32), (65521, 32), (10007, 64), (4036, 64),
                                     (3 * 4099, 32)])
 def test_register_passes(n, prec):
