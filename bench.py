@@ -6,8 +6,9 @@ Workload (the batched-uneven configuration): 20000 independent ComplexF32
 to primes (Bluestein), 30% to powers of 3, the rest left as drawn; fft/bfft
 50/50; entries N(0,1) (synthetic, generated on the device).  One step = one
 execution of the whole batch (its grouped line-pass launches issued as a
-dependency DAG over four streams; with --graph, replays of the captured
-hipGraph) with inputs already resident in HBM.
+dependency DAG over four streams; with --graph, replays of the batch's
+hipGraph, the same DAG as kernel nodes and edges) with inputs already
+resident in HBM.
 
 Reported (one JSON line):
   value      GFLOP/s = sum 5 N log2 N / step time
@@ -390,7 +391,8 @@ def main(argv=None) -> int:
     ap.add_argument("--count", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", action="store_true",
-                    help="time replays of the captured hipGraph (one serial chain) instead of the eager launch DAG")
+                    help="time replays of the batch's hipGraph (the launch DAG as kernel nodes and edges) instead of "
+                         "the eager launch DAG on four streams")
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="length of the serving replay (0: skip)")
     ap.add_argument("--no-configs", action="store_true", help="skip the single-transform configurations")
     ap.add_argument("--no-percall", action="store_true", help="skip the per-call FP64 comparison")
@@ -486,7 +488,7 @@ def main(argv=None) -> int:
             "elements": total,
             "distinct_lengths": int(np.unique(lengths).size),
             "launches_per_step": len(desc.get("phases", [])),
-            "launch": "captured hipGraph (serial chain)" if args.graph else "launch DAG on 4 streams (eager)",
+            "launch": "hipGraph of the launch DAG" if args.graph else "launch DAG on 4 streams (eager)",
             ("eager_dag_ms_per_step" if args.graph else "graph_ms_per_step"): round(other_ms, 4),
             "autotuned_radix_schedules": tuned_entries,
             "bluestein_cutoff": args.cutoff,

@@ -2680,22 +2680,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
 // for the launches that ran the previous pass of its groups), issued on
 // kStreams streams that fork from and join back into `s`; independent
 // launches (e.g. a bandwidth-bound Stockham pass and an instruction-bound
-// Bluestein pass of other lengths) overlap on the device.  Graph capture
-// uses batch_launch_serial.
-static fftahip_status batch_launch_serial(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
-  const BufPtrs bp{d_in, d_out, b->scratch_ptr};
-  for (auto& P : b->phases) {
-    if (P.reg) {
-      HIP_TRY(launch_reg_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
-                                     P.ngroups, P.mode, b->prec, bp, P.grid, s));
-      continue;
-    }
-    HIP_TRY(launch_line_pass_groups((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off,
-                                    P.ngroups, P.mode, b->prec, P.block, bp, 0, P.lds, P.grid, s));
-  }
-  return FFTAHIP_SUCCESS;
-}
-
+// Bluestein pass of other lengths) overlap on the device.
 static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
   const BufPtrs bp{d_in, d_out, b->scratch_ptr};
   const int K = fftahip_batch_s::kStreams;
@@ -2740,6 +2725,47 @@ fftahip_status fftahip_batch_execute(fftahip_batch b, const void* d_in, void* d_
   return batch_launch(b, d_in, d_out, (hipStream_t)stream);
 }
 
+// The launch DAG of a batch as a hipGraph built from kernel nodes: one node
+// per launch, an edge from every launch that ran the previous pass of one of
+// its groups (the same dependencies the eager path orders with events), so
+// independent launches stay free to overlap.  Built without stream capture.
+static fftahip_status batch_graph_build(fftahip_batch b, const void* d_in, void* d_out) {
+  const BufPtrs bp{d_in, d_out, b->scratch_ptr};
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipGraphCreate(&g, 0));
+  std::vector<hipGraphNode_t> nodes(b->phases.size(), nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < b->phases.size() && e == hipSuccess; ++i) {
+    const auto& P = b->phases[i];
+    const KernelRef k = P.reg ? reg_kernel(b->prec, P.mode) : lds_kernel(b->prec, P.block, P.mode);
+    if (!k.fn) {
+      e = hipErrorInvalidValue;
+      break;
+    }
+    LaunchArgs a((const LinePass*)b->passes.p + P.pass_off, (const uint32_t*)b->tiles.p + P.tile_off, P.ngroups, bp,
+                 P.grid);
+    hipKernelNodeParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    kp.func = const_cast<void*>(k.fn);
+    kp.gridDim = dim3(padded_grid(P.grid));
+    kp.blockDim = dim3(k.block);
+    kp.sharedMemBytes = (unsigned)P.lds;
+    kp.kernelParams = a.args;  // argument values are copied into the node
+    std::vector<hipGraphNode_t> deps;
+    for (int d : P.deps) deps.push_back(nodes[d]);
+    e = hipGraphAddKernelNode(&nodes[i], g, deps.empty() ? nullptr : deps.data(), deps.size(), &kp);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->graph = nullptr;
+    return fail(FFTAHIP_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(e));
+  }
+  b->graph_in = d_in;
+  b->graph_out = d_out;
+  return FFTAHIP_SUCCESS;
+}
+
 fftahip_status fftahip_batch_execute_graph(fftahip_batch b, const void* d_in, void* d_out, void* stream) {
   if (!b) return fail(FFTAHIP_ERR_ARGUMENT, "NULL batch");
   hipStream_t s = (hipStream_t)stream;
@@ -2749,26 +2775,8 @@ fftahip_status fftahip_batch_execute_graph(fftahip_batch b, const void* d_in, vo
     b->graph = nullptr;
   }
   if (!b->graph) {
-    hipStream_t cs;
-    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    hipGraph_t g;
-    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    // the captured graph is one chain: a multi-stream capture of the launch
-    // DAG ended the process with a host segmentation fault on this runtime
-    // (cause not found), so capture keeps the serial launch order
-    fftahip_status st = batch_launch_serial(b, d_in, d_out, cs);
-    hipError_t e = hipStreamEndCapture(cs, &g);
-    (void)hipStreamDestroy(cs);
+    fftahip_status st = batch_graph_build(b, d_in, d_out);
     if (st) return st;
-    if (e != hipSuccess) return fail(FFTAHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      b->graph = nullptr;
-      return fail(FFTAHIP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    }
-    b->graph_in = d_in;
-    b->graph_out = d_out;
   }
   HIP_TRY(hipGraphLaunch(b->graph, s));
   return FFTAHIP_SUCCESS;

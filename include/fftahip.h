@@ -163,8 +163,10 @@ typedef struct fftahip_batch_s* fftahip_batch;
 fftahip_status fftahip_batch_create(fftahip_batch* batch, int precision, int64_t count, const int64_t* lengths,
                                     const int64_t* offsets, const int* directions, int64_t bluestein_cutoff);
 fftahip_status fftahip_batch_execute(fftahip_batch batch, const void* d_in, void* d_out, void* stream);
-/* Captures one execution into a hipGraph on first use and replays it on
- * later calls with the same pointers. */
+/* Builds the batch's launch DAG as a hipGraph on first use (kernel nodes
+ * with the pass dependencies as edges, so independent launches overlap as in
+ * fftahip_batch_execute) and replays it on later calls with the same
+ * pointers. */
 fftahip_status fftahip_batch_execute_graph(fftahip_batch batch, const void* d_in, void* d_out, void* stream);
 fftahip_status fftahip_batch_describe(fftahip_batch batch, char* buf, size_t cap, size_t* needed);
 fftahip_status fftahip_batch_destroy(fftahip_batch batch);
