@@ -508,16 +508,24 @@ def main(argv=None) -> int:
     if args.serve_seconds > 0:
         from fftahip import Server
 
+        # three replays of the trace (seeds 0, 1, 2) on one warm server:
+        # p50/p99 are reported per replay and as their medians
         srv = Server(pool_bytes=64 << 30, nstreams=4)
-        st = run_serving(torch, srv, rate=50000.0, seconds=args.serve_seconds)
+        runs = [run_serving(torch, srv, rate=50000.0, seconds=args.serve_seconds, seed=sd) for sd in range(3)]
         srv.close()
+        med = lambda k: round(float(np.median([r[k] for r in runs])), 4)  # noqa: E731
         line["serving"] = {
             "workload": "Poisson 50k req/s: 70% 1-D pow2 64..4096, 20% composite 1e3..1e6, 5% prime <= 2^20, "
-                        "5% 2-D squares 256..2048; 80% ComplexF32; device-resident payloads; 64 GB pool",
-            "requests": st["requests"], "completed": st["completed"], "rejected": st["rejected"],
-            "p50_ms": round(st["p50_ms"], 4), "p99_ms": round(st["p99_ms"], 4), "max_ms": round(st["max_ms"], 4),
-            "throughput_req_s": round(st["requests_per_s"], 1), "payload_GBps": round(st["payload_gbps"], 2),
-            "pool_peak_bytes": st["pool_peak_bytes"],
+                        "5% 2-D squares 256..2048; 80% ComplexF32; device-resident payloads; 64 GB pool; "
+                        "3 replays of the trace (seeds 0-2)",
+            "requests": sum(r["requests"] for r in runs), "completed": sum(r["completed"] for r in runs),
+            "rejected": sum(r["rejected"] for r in runs),
+            "p50_ms": med("p50_ms"), "p99_ms": med("p99_ms"),
+            "p50_ms_runs": [round(r["p50_ms"], 4) for r in runs], "p99_ms_runs": [round(r["p99_ms"], 4) for r in runs],
+            "max_ms": round(max(r["max_ms"] for r in runs), 4),
+            "throughput_req_s": med("requests_per_s"), "payload_GBps": med("payload_gbps"),
+            "coalesced_requests": sum(r["coalesced"] for r in runs), "coalesced_groups": sum(r["groups"] for r in runs),
+            "pool_peak_bytes": max(r["pool_peak_bytes"] for r in runs),
         }
     if not args.no_percall:
         pc = per_call_fp64(torch, lengths, dirs)

@@ -1668,6 +1668,13 @@ struct fftahip_plan_s {
   std::vector<std::unique_ptr<LengthPlan>> lens;  // one per region dim
   Tables tables;
   std::map<std::vector<int64_t>, std::unique_ptr<DevSchedule>> scheds;  // per array shape
+  // launches of `count` vectors at arbitrary addresses (1-D plans), per
+  // count: built over offset tables whose addresses are patched per call
+  struct Many {
+    std::vector<Launch> L;
+    size_t scratch = 0;
+  };
+  std::map<int, Many> many;
   DevBuf scratch;
   DevBuf hin, hout;  // staging for execute_host
   std::string last_schedule;
@@ -3126,6 +3133,114 @@ fftahip_status plan_execute_scratch(fftahip_plan plan, int ndims, const int64_t*
   if (st) return st;
   return run_schedule(*S, plan->prec, BufPtrs{d_in, d_out, scratch}, s);
 }
+
+// Stand-ins for the offset tables of a many-vector schedule (never
+// dereferenced: replaced by each call's table addresses before upload).
+static const int64_t* const kManyTabIn = reinterpret_cast<const int64_t*>(uintptr_t(0x1000));
+static const int64_t* const kManyTabOut = reinterpret_cast<const int64_t*>(uintptr_t(0x2000));
+
+static fftahip_status many_schedule(fftahip_plan plan, int count, fftahip_plan_s::Many** out, hipStream_t s) {
+  auto it = plan->many.find(count);
+  if (it == plan->many.end()) {
+    if (plan->lens.size() != 1) return fail(FFTAHIP_ERR_ARGUMENT, "many-vector execution needs a 1-D plan");
+    LengthPlan& lp = *plan->lens[0];
+    LineSet ls;
+    ls.n[0] = 1;
+    ls.n[1] = count;
+    ls.in_tab = kManyTabIn;
+    ls.out_tab = kManyTabOut;
+    int64_t per;
+    int nbuf;
+    fftahip_status st = scratch_per_line(lp, 0, &per, &nbuf);
+    if (st) return st;
+    const size_t tables_before = plan->tables.stage.size() + plan->tables.two.size() + plan->tables.tw16.size() +
+                                 plan->tables.chirp_d.size();
+    fftahip_plan_s::Many m;
+    m.scratch = (size_t)(per * count * nbuf) * csize(plan->prec);
+    if ((st = build_launches(lp, plan->tables, ls, BufRef{BUF_IN, 0}, BufRef{BUF_OUT, 0}, BufRef{BUF_SCRATCH, 0},
+                             BufRef{BUF_SCRATCH, per * count}, plan->dir == 1, m.L, s)))
+      return st;
+    if (plan->scale != 1.0) {
+      Launch* last = nullptr;
+      for (auto& la : m.L)
+        if (la.op == OP_PASS) last = &la;
+      if (last) last->lp.scale *= plan->scale;
+    }
+    it = plan->many.emplace(count, std::move(m)).first;
+    // tables the build generated on `s` must exist before another stream
+    // uses them (usually the plan's own tables already cover the schedule)
+    if (plan->tables.stage.size() + plan->tables.two.size() + plan->tables.tw16.size() +
+            plan->tables.chirp_d.size() != tables_before)
+      HIP_TRY(hipStreamSynchronize(s));
+  }
+  *out = &it->second;
+  return FFTAHIP_SUCCESS;
+}
+
+// Device memory of one many-vector execution: the pass descriptors, the two
+// offset tables and a zero tile start, 256-byte aligned.
+static size_t many_desc_bytes(size_t nlaunch, int count) {
+  const size_t b = nlaunch * sizeof(LinePass) + 2 * (size_t)count * sizeof(int64_t) + sizeof(uint32_t);
+  return (b + 255) & ~(size_t)255;
+}
+
+fftahip_status plan_many_bytes(fftahip_plan plan, int count, size_t* scratch, size_t* desc, hipStream_t s) {
+  fftahip_plan_s::Many* m;
+  fftahip_status st = many_schedule(plan, count, &m, s);
+  if (st) return st;
+  *scratch = m->scratch;
+  *desc = many_desc_bytes(m->L.size(), count);
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status plan_execute_many(fftahip_plan plan, int count, const void* const* ins, void* const* outs,
+                                 void* scratch, void* desc, hipStream_t s) {
+  fftahip_plan_s::Many* m;
+  fftahip_status st = many_schedule(plan, count, &m, s);
+  if (st) return st;
+  const size_t esz = csize(plan->prec);
+  // element offsets from the lowest address of each side
+  uintptr_t bin = UINTPTR_MAX, bout = UINTPTR_MAX;
+  for (int i = 0; i < count; ++i) {
+    bin = std::min(bin, (uintptr_t)ins[i]);
+    bout = std::min(bout, (uintptr_t)outs[i]);
+  }
+  const size_t nl = m->L.size();
+  std::vector<unsigned char> host(many_desc_bytes(nl, count), 0);
+  LinePass* hp = reinterpret_cast<LinePass*>(host.data());
+  int64_t* tin = reinterpret_cast<int64_t*>(host.data() + nl * sizeof(LinePass));
+  int64_t* tout = tin + count;
+  for (int i = 0; i < count; ++i) {
+    const uintptr_t di = (uintptr_t)ins[i] - bin, dout = (uintptr_t)outs[i] - bout;
+    if (di % esz || dout % esz) return fail(FFTAHIP_ERR_ARGUMENT, "array addresses not element aligned");
+    tin[i] = (int64_t)(di / esz);
+    tout[i] = (int64_t)(dout / esz);
+  }
+  unsigned char* dd = (unsigned char*)desc;
+  const int64_t* d_tin = reinterpret_cast<const int64_t*>(dd + nl * sizeof(LinePass));
+  const int64_t* d_tout = d_tin + count;
+  const uint32_t* d_zero = reinterpret_cast<const uint32_t*>(d_tout + count);
+  for (size_t i = 0; i < nl; ++i) {
+    hp[i] = m->L[i].lp;
+    if (hp[i].in_tab == kManyTabIn) hp[i].in_tab = d_tin;
+    if (hp[i].out_tab == kManyTabOut) hp[i].out_tab = d_tout;
+  }
+  // pageable source: the copy is staged before the call returns
+  HIP_TRY(hipMemcpyAsync(desc, host.data(), host.size(), hipMemcpyHostToDevice, s));
+  const BufPtrs bp{(const void*)bin, (void*)bout, scratch};
+  for (size_t i = 0; i < nl; ++i) {
+    const Launch& la = m->L[i];
+    if (la.op != OP_PASS || la.grid <= 0) continue;
+    const LinePass* dpass = reinterpret_cast<const LinePass*>(dd) + i;
+    if (la.reg)
+      HIP_TRY(launch_reg_pass_groups(dpass, d_zero, 1, pass_mode(la.lp), plan->prec, bp, la.grid, s));
+    else
+      HIP_TRY(launch_line_pass_groups(dpass, d_zero, 1, pass_variant(la.lp), plan->prec, la.block, bp, 0, la.lds,
+                                      la.grid, s));
+  }
+  return FFTAHIP_SUCCESS;
+}
+
 fftahip_status set_error(fftahip_status s, const std::string& msg) { return fail(s, msg); }
 }  // namespace fftahip
 

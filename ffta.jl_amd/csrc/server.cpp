@@ -8,6 +8,8 @@
 //     hipMalloc; a request larger than the whole pool is rejected;
 //   * continuous batching: requests that arrive while others run are admitted
 //     as soon as pool space and a stream slot free up, in arrival order;
+//     requests of one 1-D plan pending together run as one coalesced group
+//     (the lines of one batch, one set of work-list launches);
 //   * overlap: micro-batches go round-robin to a set of non-blocking HIP
 //     streams, so transfers and kernels of different requests overlap;
 //   * completion: an event per micro-batch, polled by the dispatcher; the
@@ -21,6 +23,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -134,6 +137,9 @@ struct fftahip_server_s {
   // stats
   std::vector<double> lat_ms;
   uint64_t completed = 0, rejected = 0, failed = 0;
+  // requests that ran inside coalesced groups, and the number of groups
+  uint64_t coalesced = 0, groups = 0;
+  bool coalesce = true;
   double t_first = 0, t_last = 0;
   double payload_bytes = 0;
   size_t peak_pool = 0;
@@ -145,6 +151,18 @@ PlanKey key_of(const Request& r) {
   PlanKey k = {r.prec, r.dir, r.ndims};
   for (int i = 0; i < r.ndims; ++i) k.push_back(r.dims[i]);
   return k;
+}
+
+// Requests of one plan that may run as one coalesced launch set: 1-D with
+// device payloads.  A group is the largest power of two (<= kMaxGroup) of
+// such requests among the first kScan pending ones, so that each plan keeps
+// at most five many-vector schedules.
+constexpr size_t kScan = 256;
+constexpr size_t kMaxGroup = 32;
+
+bool coalescable(const Request& r) { return r.ndims == 1 && !r.host; }
+bool same_plan(const Request& a, const Request& b) {
+  return a.prec == b.prec && a.dir == b.dir && a.ndims == b.ndims && a.dims[0] == b.dims[0];
 }
 
 // Plan and scratch size for a request's shape (created on first use).  Only
@@ -165,6 +183,13 @@ fftahip_status get_plan(fftahip_server_s* S, const Request& r, fftahip_plan* p, 
     fftahip_status st = fftahip_plan_create(&plan, r.prec, r.dir, r.ndims, r.dims, r.ndims, region.data(),
                                             FFTAHIP_DEFAULT_BLUESTEIN_CUTOFF);
     if (!st) st = fftahip::plan_scratch_bytes(plan, r.ndims, r.dims, &sb);
+    // the schedules of coalesced groups of this plan, built now rather than
+    // on first use inside admission
+    if (!st && coalescable(r))
+      for (size_t K = 2; K <= kMaxGroup && !st; K *= 2) {
+        size_t a = 0, b = 0;
+        st = fftahip::plan_many_bytes(plan, (int)K, &a, &b, nullptr);
+      }
     if (st && plan) fftahip_plan_destroy(plan);
     lk.lock();
     if (st) return st;
@@ -253,8 +278,8 @@ void worker_loop(fftahip_server_s* S) {
         S->cv_done.notify_all();
         continue;
       }
-      const size_t need = need_bytes(*r, scratch);
-      if (need > S->arena->size()) {
+      const size_t need1 = need_bytes(*r, scratch);
+      if (need1 > S->arena->size()) {
         S->pending.pop_front();
         set_error(FFTAHIP_ERR_CAPACITY, "request exceeds the device pool");
         finish(S, r, FFTAHIP_ERR_CAPACITY, now_s());
@@ -266,6 +291,32 @@ void worker_loop(fftahip_server_s* S) {
       for (int i = 1; i < (int)S->streams.size(); ++i)
         if (S->stream_inflight[i] < S->stream_inflight[sidx]) sidx = i;
       if (S->stream_inflight[sidx] >= S->max_per_stream) break;
+      hipStream_t s = S->streams[sidx];
+      // coalesce the requests of r's plan pending behind it
+      std::vector<Request*> group{r};
+      if (S->coalesce && coalescable(*r)) {
+        size_t scanned = 0;
+        for (auto it = S->pending.begin() + 1;
+             it != S->pending.end() && scanned < kScan && group.size() < kMaxGroup; ++it, ++scanned)
+          if (coalescable(**it) && same_plan(**it, *r)) group.push_back(*it);
+      }
+      size_t K = 1;
+      while (K * 2 <= group.size()) K *= 2;
+      group.resize(K);
+      size_t need = need1, scr_many = 0, desc_many = 0;
+      if (K > 1) {
+        // the first use of a group size builds its schedule (and may wait
+        // for table kernels): without the lock, as plan creation
+        lk.unlock();
+        const fftahip_status ms = fftahip::plan_many_bytes(plan, (int)K, &scr_many, &desc_many, s);
+        lk.lock();
+        if (ms) {
+          K = 1;
+          group.resize(1);
+        } else {
+          need = ((scr_many + 255) & ~(size_t)255) + desc_many;
+        }
+      }
       size_t off = 0;
       if (!S->arena->alloc(need, &off)) break;  // wait for memory to free up
       S->peak_pool = std::max(S->peak_pool, S->arena->used());
@@ -273,14 +324,28 @@ void worker_loop(fftahip_server_s* S) {
       f.stream = sidx;
       f.off = off;
       f.bytes = need;
-      f.reqs.push_back(r);
+      f.reqs = group;
       S->pending.pop_front();
-      hipStream_t s = S->streams[sidx];
-      st = issue(S, r, plan, scratch, (char*)S->pool + off, s);
+      for (size_t g = 1; g < K; ++g) S->pending.erase(std::find(S->pending.begin(), S->pending.end(), group[g]));
+      if (K > 1) {
+        std::vector<const void*> ins(K);
+        std::vector<void*> outs(K);
+        for (size_t g = 0; g < K; ++g) {
+          ins[g] = group[g]->in;
+          outs[g] = group[g]->out;
+        }
+        char* base = (char*)S->pool + off;
+        st = fftahip::plan_execute_many(plan, (int)K, ins.data(), outs.data(), scr_many ? base : nullptr,
+                                        base + ((scr_many + 255) & ~(size_t)255), s);
+        S->coalesced += K;
+        S->groups++;
+      } else {
+        st = issue(S, r, plan, scratch, (char*)S->pool + off, s);
+      }
       if (st) {
         (void)hipStreamSynchronize(s);
         S->arena->release(off, need);
-        finish(S, r, st, now_s());
+        for (Request* q : group) finish(S, q, st, now_s());
         S->cv_done.notify_all();
         continue;
       }
@@ -312,6 +377,10 @@ fftahip_status fftahip_server_create(fftahip_server* out, int64_t pool_bytes, in
   if (nstreams <= 0) nstreams = 4;
   auto S = std::make_unique<fftahip_server_s>();
   (void)hipGetDevice(&S->device);
+  {
+    const char* v = std::getenv("FFTAHIP_COALESCE");
+    S->coalesce = !(v && v[0] == '0');
+  }
   hipError_t e = hipMalloc(&S->pool, (size_t)pool_bytes);
   if (e != hipSuccess) return set_error(FFTAHIP_ERR_ALLOC, std::string("server pool: ") + hipGetErrorString(e));
   S->arena = std::make_unique<Arena>((size_t)pool_bytes);
@@ -446,6 +515,8 @@ fftahip_status fftahip_server_stats_get(fftahip_server s, fftahip_server_stats* 
   out->payload_gbps = span > 0 ? s->payload_bytes / span / 1e9 : 0;
   out->pool_peak_bytes = (int64_t)s->peak_pool;
   out->plans = (int64_t)s->plans.size();
+  out->coalesced = (int64_t)s->coalesced;
+  out->groups = (int64_t)s->groups;
   return FFTAHIP_SUCCESS;
 }
 
@@ -454,6 +525,7 @@ fftahip_status fftahip_server_reset_stats(fftahip_server s) {
   std::lock_guard<std::mutex> g(s->mu);
   s->lat_ms.clear();
   s->completed = s->rejected = s->failed = 0;
+  s->coalesced = s->groups = 0;
   s->t_first = s->t_last = 0;
   s->payload_bytes = 0;
   s->peak_pool = s->arena->used();

@@ -97,7 +97,8 @@ class ServerStats(ctypes.Structure):
                 ("pending", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("seconds", ctypes.c_double),
                 ("requests_per_s", ctypes.c_double), ("payload_gbps", ctypes.c_double),
-                ("pool_peak_bytes", ctypes.c_int64), ("plans", ctypes.c_int64)]
+                ("pool_peak_bytes", ctypes.c_int64), ("plans", ctypes.c_int64), ("coalesced", ctypes.c_int64),
+                ("groups", ctypes.c_int64)]
 
 
 _lib = None

@@ -175,14 +175,18 @@ fftahip_status fftahip_batch_destroy(fftahip_batch batch);
  * A request loop with continuous batching (no reference counterpart: FFTA
  * is a library call; this serves many fft/bfft calls concurrently).  One
  * dispatcher thread admits submitted requests in arrival order against a
- * device memory pool of pool_bytes (allocated once at creation; plan scratch
- * and host-payload staging come from it), as soon as memory and a stream
- * slot are free, and runs them on `nstreams` non-blocking HIP streams so
- * that their work overlaps.  Plans are cached per (precision, direction,
- * shape).  A request larger than the whole pool fails with
- * FFTAHIP_ERR_CAPACITY.  Payloads are device pointers, or host pointers when
- * host_buffers != 0 (staged through the pool).  Latency runs from arrival
- * (submission, or the scheduled time for replay) to completion. */
+ * device memory pool of pool_bytes (allocated once at creation; plan scratch,
+ * descriptors and host-payload staging come from it), as soon as memory and
+ * a stream slot are free, and runs them on `nstreams` non-blocking HIP
+ * streams so that their work overlaps.  Requests of one plan (1-D, device
+ * payloads, same precision, direction and length) pending together are
+ * coalesced: up to 32 of them run as the lines of one batch, one set of
+ * work-list launches.  Plans are cached per (precision, direction, shape)
+ * and built outside the dispatcher's lock.  A request larger than the whole
+ * pool fails with FFTAHIP_ERR_CAPACITY.  Payloads are device pointers, or
+ * host pointers when host_buffers != 0 (staged through the pool).  Latency
+ * runs from arrival (submission, or the scheduled time for replay) to
+ * completion.  Env FFTAHIP_COALESCE=0 at creation turns coalescing off. */
 typedef struct fftahip_server_s* fftahip_server;
 
 typedef struct {
@@ -193,6 +197,8 @@ typedef struct {
   double payload_gbps;     /* (input + output bytes) / seconds */
   int64_t pool_peak_bytes;
   int64_t plans;
+  int64_t coalesced;       /* requests that ran inside coalesced groups */
+  int64_t groups;          /* coalesced groups (one set of launches each) */
 } fftahip_server_stats;
 
 fftahip_status fftahip_server_create(fftahip_server* server, int64_t pool_bytes, int nstreams);

@@ -108,3 +108,56 @@ def test_replay_short_trace(torch_cuda):
     assert res["completed"] == res["requests"] and res["failed"] == 0
     assert 0 < res["p50_ms"] <= res["p99_ms"]
     srv.close()
+
+
+def test_coalesced_groups_match_alone(torch_cuda):
+    """Requests of one 1-D plan that pile up behind others run as coalesced
+    groups (one set of work-list launches for up to 32 of them); each result
+    matches the oracle and the same transform run alone."""
+    from fftahip import Server
+
+    torch = torch_cuda
+    srv = Server(pool_bytes=4 << 30, nstreams=2)
+    rng = np.random.default_rng(21)
+    shapes = [((4096,), 32, -1), ((1000,), 64, 1), ((10007,), 32, 1), ((262144,), 32, -1), ((97 * 8,), 32, -1),
+              ((12289,), 64, -1)]
+    reqs = []
+    # a large 2-D request first keeps the streams busy while the burst queues
+    big = np.asfortranarray((rng.standard_normal((2048, 2048)) + 1j * rng.standard_normal((2048, 2048)))
+                            .astype(np.complex64))
+    tb, _ = to_device(torch, big)
+    tby, _ = empty_device(torch, big.shape, np.complex64)
+    for rep in range(40):
+        for shape, prec, d in shapes:
+            dt = np.complex64 if prec == 32 else np.complex128
+            x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dt)
+            tx, _ = to_device(torch, x)
+            ty, _ = empty_device(torch, shape, dt)
+            reqs.append((x, tx, ty, shape, prec, d))
+    torch.cuda.synchronize()
+    print(f"case serving coalesced burst of {len(reqs)} requests", flush=True)
+    tickets = [srv.submit(32, -1, big.shape, tb.data_ptr(), tby.data_ptr())]
+    tickets += [srv.submit(prec, d, shape, tx.data_ptr(), ty.data_ptr()) for (x, tx, ty, shape, prec, d) in reqs]
+    for tk in tickets:
+        srv.wait(tk)
+    st = srv.stats()
+    print(f"  coalesced {st['coalesced']} requests in {st['groups']} groups", flush=True)
+    assert st["completed"] == len(tickets) and st["failed"] == 0
+    assert st["groups"] > 0 and st["coalesced"] >= 2 * st["groups"]
+    alone = {}
+    exact = 0
+    for (x, tx, ty, shape, prec, d) in reqs:
+        y = to_host(torch, ty, shape)
+        n = int(np.prod(shape))
+        ref = oracle.fft(x.astype(np.complex128), d)
+        assert oracle.relerr(y, ref) <= tol(n, prec), (shape, prec, d)
+        key = (shape, prec, d)
+        if key not in alone:
+            alone[key] = True
+        a = _alone(torch, x, prec, d)
+        assert oracle.relerr(y, a.astype(np.complex128)) <= tol(n, prec)
+        exact += int(np.array_equal(y, a))
+    print(f"  {exact}/{len(reqs)} bitwise identical to the transform run alone", flush=True)
+    yb = to_host(torch, tby, big.shape)
+    assert oracle.relerr(yb, oracle.fftn(big.astype(np.complex128), None, -1)) <= tol(big.size, 32)
+    srv.close()

@@ -10,9 +10,9 @@
 * Serving: the Poisson trace at 50k req/s for 0.5 s (70% 1-D pow2 64..4096,
   20% composite 1e3..1e6, 5% prime <= 2^20, 5% 2-D squares 256..2048; 80%
   ComplexF32) replayed through the request loop with a 64 GB pool.  Every
-  request writes its own output buffer; each result must equal bit for bit
-  the first result of its (shape, precision, direction), and that one is
-  checked against the FP64 oracle.
+  request writes its own output buffer; each result must match the first
+  result of its (shape, precision, direction) within the tolerance, and that
+  one is checked against the FP64 oracle.
 """
 from __future__ import annotations
 
@@ -109,18 +109,28 @@ def test_serving_config_replay(torch_cuda):
     print(f"  completed {st['completed']}/{len(tickets)}  p50 {st['p50_ms']:.4f} ms  p99 {st['p99_ms']:.4f} ms  "
           f"{st['requests_per_s']:.0f} req/s", flush=True)
     assert st["completed"] == len(tickets) and st["failed"] == 0 and st["rejected"] == 0
+    print(f"  coalesced {st['coalesced']} requests in {st['groups']} groups", flush=True)
     torch.cuda.synchronize()
+    # every result against the first result of its (shape, precision,
+    # direction), on the device: a request that ran inside a coalesced group
+    # may round differently from one that ran alone, within the tolerance
     first = {}
-    mismatched = []
+    bad, exact = [], 0
     for i, (sh, p, d) in enumerate(zip(shapes, prec.tolist(), dirs.tolist())):
         key = (sh, p, d)
         if key not in first:
             first[key] = i
-        elif not torch.equal(outs[i], outs[first[key]]):
-            mismatched.append((i, key))
-    assert not mismatched, mismatched[:10]
-    print(f"  {len(first)} distinct (shape, precision, direction); every result equals its first bitwise",
-          flush=True)
+            continue
+        a, b = outs[i], outs[first[key]]
+        if torch.equal(a, b):
+            exact += 1
+            continue
+        rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+        if not rel <= tol(int(np.prod(sh)), p):
+            bad.append((i, key, rel))
+    assert not bad, bad[:10]
+    print(f"  {len(first)} distinct (shape, precision, direction); {exact}/{len(shapes) - len(first)} repeats "
+          f"bitwise equal to the first, all within tolerance", flush=True)
 
     def check(key):
         sh, p, d = key
