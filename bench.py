@@ -239,7 +239,8 @@ PER_CALL_CONFIGS = [
 ]
 
 
-def per_call_configs(torch, cpu: bool = True, seed: int = 7, cpu_budget_s: float = 3.0):
+def per_call_configs(torch, cpu: bool = True, seed: int = 7, cpu_budget_s: float = 3.0, reps_fixed: int = 0,
+                     only=None):
     """The single-transform configurations: GPU time per mul! with the arrays
     resident in HBM (HIP events on the launch stream), error against the FP64
     oracle, and the oracle in the plan's precision timed single-threaded on
@@ -250,6 +251,8 @@ def per_call_configs(torch, cpu: bool = True, seed: int = 7, cpu_budget_s: float
     rows = []
     stream = torch.cuda.current_stream()
     for name, shape, prec, direction, dist in PER_CALL_CONFIGS:
+        if only and not any(o == f"{'x'.join(map(str, shape))}:{prec}" for o in only):
+            continue
         rng = np.random.default_rng(seed)
         n = int(np.prod(shape))
         if dist == "uniform":
@@ -277,7 +280,7 @@ def per_call_configs(torch, cpu: bool = True, seed: int = 7, cpu_budget_s: float
         e1.record(stream)
         torch.cuda.synchronize()
         est = max(e0.elapsed_time(e1), 1e-3)
-        reps = int(max(5, min(200, 200.0 / est)))
+        reps = reps_fixed if reps_fixed > 0 else int(max(5, min(200, 200.0 / est)))
         e0.record(stream)
         for _ in range(reps):
             run()
@@ -396,13 +399,19 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-seconds", type=float, default=0.5, help="length of the serving replay (0: skip)")
     ap.add_argument("--no-configs", action="store_true", help="skip the single-transform configurations")
     ap.add_argument("--no-percall", action="store_true", help="skip the per-call FP64 comparison")
+    ap.add_argument("--only-configs", action="store_true",
+                    help="run only the single-transform configurations (profiling their kernels)")
+    ap.add_argument("--config-reps", type=int, default=0, help="timed calls per configuration (0: about 200 ms)")
+    ap.add_argument("--config", action="append", default=None,
+                    help="restrict --only-configs to a shape:precision, e.g. 4096x4096:32 (repeatable)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the workload's in-LDS radix schedules on this device, write TUNING_DB and exit")
     ap.add_argument("--no-tuning", action="store_true", help="ignore TUNING_DB (default radix rules)")
     ap.add_argument("--cutoff", type=int, default=73,
                     help="BLUESTEIN_CUTOFF of the batch (FFTA's keyword; default 73 as FFTA)")
     ap.add_argument("--probe-copy", action="store_true",
-                    help="after the steps, copy 1 GiB once (calibrates HBM byte counters under rocprofv3 --pmc)")
+                    help="after the steps, run one line pass of known bytes (256 MiB each way; calibrates the HBM "
+                         "byte counters under rocprofv3 --pmc)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -414,6 +423,10 @@ def main(argv=None) -> int:
     from fftahip import autotune
     from fftahip.plan import Batch
 
+    if args.only_configs:
+        rows = per_call_configs(torch, cpu=not args.no_cpu, reps_fixed=args.config_reps, only=args.config)
+        print(json.dumps({"per_call_configs": rows}), flush=True)
+        return 0
     lengths, dirs = batched_uneven(args.count)
     if args.autotune:
         t0 = time.perf_counter()
@@ -455,12 +468,16 @@ def main(argv=None) -> int:
     other_ms = timed(not args.graph, args.steps, args.warmup)
 
     if args.probe_copy:
-        a = torch.zeros(1 << 27, dtype=torch.complex64, device="cuda")
+        # a launch of known HBM bytes: one in-LDS line pass over 32768
+        # transforms of length 1024 (256 MiB read, 256 MiB written, plus a
+        # 16 KiB twiddle table)
+        kb = Batch(np.full(32768, 1024, dtype=np.int64), np.full(32768, -1, dtype=np.int32), precision=32)
+        a = torch.randn(32768 * 1024, dtype=torch.complex64, device="cuda")
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        b.copy_(a)
+        kb.execute_device(a.data_ptr(), b.data_ptr(), stream.cuda_stream)
         torch.cuda.synchronize()
-        del a, b
+        del a, b, kb
 
     flops = flops_of(lengths)
     bytes_step = 2.0 * total * 8
