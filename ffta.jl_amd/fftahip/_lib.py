@@ -10,7 +10,9 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)                 # ffta.jl_amd/
-LIB_PATH = os.path.join(PKG_ROOT, "lib", "libfftahip.so")
+# FFTAHIP_LIB selects another build of the same library (the host
+# AddressSanitizer build, lib/libfftahip_asan.so, in the sanitizer test)
+LIB_PATH = os.environ.get("FFTAHIP_LIB") or os.path.join(PKG_ROOT, "lib", "libfftahip.so")
 
 SUCCESS = 0
 ERR_DIMENSION_MISMATCH = 1
