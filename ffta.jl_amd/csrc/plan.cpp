@@ -544,10 +544,12 @@ static int64_t lds_budget() {
 }
 // Adjacent lines per workgroup for strided passes: 128-byte segments.
 static int64_t min_lines(int prec) { return prec == 32 ? 16 : 8; }
-// LDS per workgroup for strided (multi-pass) tiles; small enough for
-// two or more resident workgroups per CU.
+// LDS per workgroup for strided (multi-pass) tiles: 40 KB, so that three
+// or more workgroups share a CU and hide each other's global-memory latency
+// (measured on the batched workload: 3.62 ms per step against 3.77 ms with
+// 64 KB tiles; 32 KB and below split lines into more passes).
 static int64_t lds_budget();
-static int64_t pass_budget() { return std::min<int64_t>(env_i64("FFTAHIP_PASS_LDS", 65536), lds_budget()); }
+static int64_t pass_budget() { return std::min<int64_t>(env_i64("FFTAHIP_PASS_LDS", 40960), lds_budget()); }
 static int64_t single_max(int prec) {
   // room for the 1-in-16 LDS padding
   int64_t m = lds_budget() / (int64_t)csize(prec) * 16 / 17;
@@ -1604,7 +1606,11 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
     // three fused passes: among the splittable 5-smooth M up to 15% above
     // the first, the least M * (row stages + 3) (the column passes and the
     // row pass's own load, spectrum and store phases cost about three
-    // stages per element); env FFTAHIP_BLUE_COST=0 takes the first
+    // stages per element), rows that are not a power of two weighted by
+    // 1.5 (their radix-3/5 stages and partly filled workgroups ran at
+    // 1.4-2.2 TB/s against 3.0 TB/s for 4096-point rows; 3.56 against
+    // 3.61 ms per batched step); env FFTAHIP_BLUE_COST=0 takes the first,
+    // FFTAHIP_BLUE_NP2 sets the weight in percent
     const bool costed = env_i64("FFTAHIP_BLUE_COST", 1) != 0;
     double best_cost = 1e300;
     int64_t limit = chosen ? 0 : 4 * (2 * N - 1);  // one kernel: no three-pass search
@@ -1620,7 +1626,8 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
       }
       Inner row;
       make_inner(b, row, prec);
-      const double cost = (double)M * ((double)row.rad.size() + 3.0);
+      double cost = (double)M * ((double)row.rad.size() + 3.0);
+      if (b & (b - 1)) cost *= (double)env_i64("FFTAHIP_BLUE_NP2", 150) / 100.0;
       if (cost < best_cost) {
         best_cost = cost;
         chosen = M;
