@@ -1342,14 +1342,23 @@ struct DevSchedule {
   size_t scratch_need = 0;
 };
 
-static fftahip_status upload_schedule(DevSchedule& S) {
+// Uploads a newly built schedule and waits for everything it depends on:
+// the table kernels its build launched on `s` and the descriptor copies.  A
+// cached schedule may next run on any stream (the serving loop spreads one
+// plan's requests over several non-blocking streams, which do not order
+// themselves after the null stream), so nothing of the build may still be
+// in flight when this returns.
+static fftahip_status upload_schedule(DevSchedule& S, hipStream_t s) {
   std::vector<LinePass> host;
   for (const Launch& la : S.L) host.push_back(la.lp);
   fftahip_status st = S.passes.ensure(std::max<size_t>(1, host.size()) * sizeof(LinePass));
   if (st) return st;
   if ((st = S.zero.ensure(sizeof(uint32_t)))) return st;
+  HIP_TRY(hipStreamSynchronize(s));
   if (!host.empty()) HIP_TRY(hipMemcpy(S.passes.p, host.data(), host.size() * sizeof(LinePass), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(S.zero.p, 0, sizeof(uint32_t)));
+  const uint32_t zero = 0;
+  HIP_TRY(hipMemcpy(S.zero.p, &zero, sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipDeviceSynchronize());
   return FFTAHIP_SUCCESS;
 }
 
@@ -1395,7 +1404,7 @@ static fftahip_status fft64_device(const void* in, void* out, int64_t n, hipStre
   if ((st = build_launches(sub, T64, ls, BufRef{BUF_IN, 0}, BufRef{BUF_OUT, 0}, BufRef{BUF_SCRATCH, 0},
                            BufRef{BUF_SCRATCH, per}, false, S.L, s)))
     return st;
-  if ((st = upload_schedule(S))) return st;
+  if ((st = upload_schedule(S, s))) return st;
   if ((st = run_schedule(S, 64, BufPtrs{in, out, s0.p}, s))) return st;
   // the tables and scratch above are freed on return
   HIP_TRY(hipStreamSynchronize(s));
@@ -2000,7 +2009,7 @@ static fftahip_status plan_shape_schedule(fftahip_plan plan, int ndims, const in
     auto S = std::make_unique<DevSchedule>();
     fftahip_status st = plan_schedule(plan, ndims, dims, S->L, &S->scratch_need, s);
     if (st) return st;
-    if ((st = upload_schedule(*S))) return st;
+    if ((st = upload_schedule(*S, s))) return st;
     it = plan->scheds.emplace(key, std::move(S)).first;
   }
   *out = it->second.get();
@@ -2026,17 +2035,11 @@ fftahip_status fftahip_execute(fftahip_plan plan, int ndims, const int64_t* dims
   int64_t total = 1;
   for (int i = 0; i < ndims; ++i) total *= dims[i];
   if (total == 0) return FFTAHIP_SUCCESS;
-  std::vector<int64_t> key(dims, dims + ndims);
-  auto it = plan->scheds.find(key);
-  if (it == plan->scheds.end()) {
-    auto S = std::make_unique<DevSchedule>();
-    fftahip_status st = plan_schedule(plan, ndims, dims, S->L, &S->scratch_need, s);
-    if (st) return st;
-    if ((st = upload_schedule(*S))) return st;
-    it = plan->scheds.emplace(key, std::move(S)).first;
-  }
-  DevSchedule& S = *it->second;
-  fftahip_status st = plan->scratch.ensure(S.scratch_need);
+  DevSchedule* Sp;
+  fftahip_status st = plan_shape_schedule(plan, ndims, dims, &Sp, s);
+  if (st) return st;
+  DevSchedule& S = *Sp;
+  st = plan->scratch.ensure(S.scratch_need);
   if (st) return st;
   return run_schedule(S, plan->prec, BufPtrs{d_in, d_out, plan->scratch.p}, s);
 }
@@ -2343,8 +2346,8 @@ fftahip_status fftahip_rplan_create(fftahip_rplan* plan, int precision, int dire
       p->post = real_geom(P, n, Q, P, P * n, P, P * n);
     }
   }
-  if ((st = upload_schedule(p->S1))) return st;
-  if ((st = upload_schedule(p->S2))) return st;
+  if ((st = upload_schedule(p->S1, s))) return st;
+  if ((st = upload_schedule(p->S2, s))) return st;
   *plan = p.release();
   return FFTAHIP_SUCCESS;
 }
