@@ -386,7 +386,8 @@ def per_call_fp64(torch, lengths, dirs, sample: int = 2000, seed: int = 3):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "line_pass_batched_uneven_c32_traffic.json")
-# radix schedules tuned on an MI355X for this workload (python bench.py --autotune)
+# radix schedules tuned on an MI355X for this workload and strided tile shapes
+# for the single-transform configurations (python bench.py --autotune)
 TUNING_DB = os.path.join(ROOT, "ffta.jl_amd", "tuning", "mi355x_batched_uneven_c32.txt")
 
 
@@ -424,7 +425,10 @@ def main(argv=None) -> int:
                     help="restrict --only-configs to a shape:precision, e.g. 4096x4096:32 (repeatable)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the workload's in-LDS radix schedules on this device, write TUNING_DB and exit")
-    ap.add_argument("--no-tuning", action="store_true", help="ignore TUNING_DB (default radix rules)")
+    ap.add_argument("--autotune-configs", action="store_true",
+                    help="tune the strided tile shapes of the single-transform configurations on this device "
+                         "(keeping TUNING_DB's radix schedules), write TUNING_DB and exit")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore TUNING_DB (default radix rules and tiles)")
     ap.add_argument("--cutoff", type=int, default=73,
                     help="BLUESTEIN_CUTOFF of the batch (FFTA's keyword; default 73 as FFTA)")
     ap.add_argument("--probe-copy", action="store_true",
@@ -441,21 +445,31 @@ def main(argv=None) -> int:
     from fftahip import autotune
     from fftahip.plan import Batch
 
+    tuned_entries = 0
+    if args.autotune or args.autotune_configs:
+        t0 = time.perf_counter()
+        res = {}
+        if args.autotune:
+            lengths, dirs = batched_uneven(args.count)
+            res = autotune.tune_batch(lengths, dirs, 32)
+        elif os.path.exists(TUNING_DB):
+            autotune.load_table(TUNING_DB)
+        # strided tile shapes of the single-transform configurations
+        res["config_tiles"] = {}
+        for shape, prec in sorted({(shape, prec) for _, shape, prec, _, _ in PER_CALL_CONFIGS}):
+            res["config_tiles"][f"{'x'.join(map(str, shape))}:{prec}"] = [
+                autotune.tune_shape(shape, prec), [autotune.tile(prec, n) for n in shape]]
+        autotune.save(TUNING_DB)
+        print(json.dumps({"autotune": res, "seconds": round(time.perf_counter() - t0, 1), "db": TUNING_DB}),
+              flush=True)
+        return 0
+    if not args.no_tuning and os.path.exists(TUNING_DB):
+        tuned_entries = autotune.load_table(TUNING_DB)
     if args.only_configs:
         rows = per_call_configs(torch, cpu=not args.no_cpu, reps_fixed=args.config_reps, only=args.config)
         print(json.dumps({"per_call_configs": rows}), flush=True)
         return 0
     lengths, dirs = batched_uneven(args.count)
-    if args.autotune:
-        t0 = time.perf_counter()
-        res = autotune.tune_batch(lengths, dirs, 32)
-        autotune.save(TUNING_DB)
-        print(json.dumps({"autotune": res, "seconds": round(time.perf_counter() - t0, 1), "db": TUNING_DB}),
-              flush=True)
-        return 0
-    tuned_entries = 0
-    if not args.no_tuning and os.path.exists(TUNING_DB):
-        tuned_entries = autotune.load_table(TUNING_DB)
     t0 = time.perf_counter()
     batch = Batch(lengths, dirs, precision=32, cutoff=args.cutoff)
     plan_s = time.perf_counter() - t0

@@ -1,5 +1,5 @@
 // codelets.h -- forward in-register DFTs of radix 2, 3, 4, 5, 7, 8, 9, 11,
-// 13 and 16 (X[k] = sum_n x[n] exp(-2 pi i n k / R)), and of every 13-smooth
+// 13, 16, 17, 19 and 23 (X[k] = sum_n x[n] exp(-2 pi i n k / R)), and of every 13-smooth
 // composite length as a two-factor split of those.  The backward
 // transform is obtained by conjugating on load and on store, so only the
 // forward sign exists on the device.
@@ -176,7 +176,7 @@ __device__ __forceinline__ void dft_any(cx<R>* v) {
     dft_split<R, 3, 3>(v);
   } else if constexpr (P == 16) {
     dft_split<R, 4, 4>(v);
-  } else if constexpr (P == 5 || P == 7 || P == 11 || P == 13) {
+  } else if constexpr (P == 5 || P == 7 || P == 11 || P == 13 || P == 17 || P == 19 || P == 23) {
     dft_prime<R, P>(v);
   } else {
     // 13-smooth composite (register line passes): two-factor split

@@ -6,6 +6,16 @@
 namespace fftahip {
 
 constexpr int kMaxStages = 12;      // radix stages inside one in-LDS line FFT
+
+// Prime radices above 16 that run as compile-time codelets (dft_prime, one
+// butterfly per thread) in the FP32 direct-DFT kernel variants (in FP64 their
+// values exceed the register budget); every other prime >= 17 runs the
+// table-driven direct-DFT stage, whose roots follow the stage twiddles in the
+// stage table.  prec: 32 or 64.
+__host__ __device__ constexpr bool prime_codelet(int r, int prec) {
+  return prec == 32 && (r == 17 || r == 19 || r == 23);
+}
+__host__ __device__ constexpr bool big_stage(int r, int prec) { return r >= 17 && !prime_codelet(r, prec); }
 constexpr int kMaxLines = 256;      // lines per workgroup, at most
 constexpr int kRegBlock = 256;      // lines (threads) per workgroup of a register line pass
 // per-line offset tables of a line-pass tile, kept in LDS after its lines:
@@ -106,6 +116,12 @@ struct LinePass {
   // 1: the spectrum product is applied by the second sweep's first radix
   // stage as it reads its inputs (no separate spectrum phase)
   int32_t mid_in_stage;
+  // 1: the first sweep's last stage, the spectrum product and the second
+  // sweep's first stage run as one stage (first and last radix equal)
+  int32_t mid_fused;
+  // 1: every line of a tile lies within one wavefront during the sweeps
+  // (threads per line divide 64), so their stages synchronise per wave
+  int32_t wave_sync;
   FastDiv divC, divL;       // by C and by L
   FastDiv divR;             // by block / C
   int64_t nlines;           // < 2^31

@@ -12,6 +12,10 @@ namespace fftahip {
 
 // ------------------------------------------------------------------ helpers
 
+// 32 or 64: the precision argument of prime_codelet / big_stage
+template <class Rt>
+constexpr int kPrec = sizeof(Rt) == 4 ? 32 : 64;
+
 template <class R>
 __device__ __forceinline__ cx<R> from_d2(double2 w) {
   return {(R)w.x, (R)w.y};
@@ -94,6 +98,22 @@ __device__ __forceinline__ double2 chirp_root(uint64_t m, int64_t N) {
 __device__ __forceinline__ int pad_shift(int sh) { return sh ? sh : 31; }
 __device__ __forceinline__ int pad_idx(int i, int she) { return i + (i >> she); }
 
+// Barrier between the LDS phases of a radix sweep.  When every line of the
+// tile lies within one wavefront (threads per line divide 64: the planner's
+// wave_sync), only the lanes of one wave exchange data, and one wave's LDS
+// instructions execute in program order, so ordering the wave's own LDS
+// accesses (wavefront-scope fences around a wave barrier) replaces the
+// workgroup barrier: the waves of a tile then run their sweeps independently.
+__device__ __forceinline__ void line_sync(bool wave) {
+  if (wave) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One radix-RAD Stockham stage over a line in LDS.  Stage twiddles come from
 // a per-stage table: stage s (Ns = product of earlier radices) keeps
 // w_{Ns*RAD}^{t*k} for k < Ns, t = 1..RAD-1 at tw[(Ns-1) + k*(RAD-1) + t-1],
@@ -104,13 +124,16 @@ __device__ __forceinline__ int pad_idx(int i, int she) { return i + (i >> she); 
 template <class Rt, int RAD, int EPT, bool FUSE, class LdF, class StF>
 __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl, int L, int Ns, FastDiv nsd,
                                       const cx<Rt>* __restrict__ tw, int sh, bool gload, bool gstore, const LdF& ld,
-                                      const StF& st, const cx<Rt>* __restrict__ pm, int pms) {
-  constexpr int NB = EPT / RAD;
+                                      const StF& st, const cx<Rt>* __restrict__ pm, int pms, bool ws) {
+  // butterflies per thread (a prime codelet above EPT points: one)
+  constexpr int NB = EPT / RAD > 0 ? EPT / RAD : 1;
   const int Lr = L / RAD;
   const int she = pad_shift(sh);
   const int pmask = sh ? (1 << sh) - 1 : 0;
   const bool rstep = (Lr & pmask) == 0;
-  const bool wstep = (Ns & pmask) == 0;
+  // the first stage (Ns = 1) writes RAD consecutive positions j*RAD + t,
+  // which stay inside one padding block when RAD divides the block size
+  const bool wstep = (Ns & pmask) == 0 || (Ns == 1 && ((pmask + 1) % RAD) == 0);
   const int rstride = Lr + (sh ? (Lr >> sh) : 0);
   const int wstride = Ns + (sh ? (Ns >> sh) : 0);
   cx<Rt> v[NB][RAD];
@@ -176,7 +199,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       dft_any<Rt, RAD>(v[b]);
     }
   }
-  __syncthreads();
+  line_sync(ws);
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = jl + b * Tl;
@@ -197,7 +220,7 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       }
     }
   }
-  __syncthreads();
+  line_sync(ws);
 }
 
 // One radix-P stage for a prime P >= 17 over a line in LDS: the direct DFT
@@ -219,7 +242,8 @@ constexpr int kBigHold = 2;
 
 template <class Rt>
 __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int Tl, int L, int P, int Ns, FastDiv nsd,
-                                          const cx<Rt>* __restrict__ tw, const cx<Rt>* __restrict__ roots, int sh) {
+                                          const cx<Rt>* __restrict__ tw, const cx<Rt>* __restrict__ roots, int sh,
+                                          bool ws) {
   const int Lr = L / P;
   const int H = (P - 1) >> 1;
   const int G = (H + kBigKP - 1) / kBigKP;
@@ -281,7 +305,7 @@ __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int
     og[h] = g;
     obase[h] = (int)(q * (uint32_t)Ns * (uint32_t)P) + k;
   }
-  __syncthreads();
+  line_sync(ws);
 #pragma unroll
   for (int h = 0; h < kBigHold; ++h) {
     if (og[h] < 0) continue;
@@ -295,38 +319,120 @@ __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int
     }
     if (og[h] == 0) line[pad_idx(base, she)] = out[h][2 * kBigKP];
   }
-  __syncthreads();
+  line_sync(ws);
+}
+
+// The boundary between the two sweeps of a convolution line, in one stage:
+// the last radix-RAD stage of the first sweep (Ns = L / RAD: butterfly j
+// reads positions j + t*L/RAD with its stage twiddles and produces the
+// outputs at the same positions), the spectrum product v = conj(v * pm[i *
+// pms]) at those positions, and the first stage of the second sweep (Ns = 1,
+// no twiddles, the same radix over the same positions), which writes
+// positions j*RAD + t.  One LDS round trip and two barriers fewer than
+// running the two stages apart.  With rader_dc the first sweep's DC term
+// (position 0) is kept in the slot rader_dc before the product.
+template <class Rt, int RAD, int EPT>
+__device__ __forceinline__ void stage_mid(cx<Rt>* line, int jl, bool active, int Tl, int L,
+                                          const cx<Rt>* __restrict__ tw, int sh, const cx<Rt>* __restrict__ pm, int pms,
+                                          int rader_dc, bool ws) {
+  constexpr int NB = EPT / RAD;
+  const int Lr = L / RAD;
+  const int Ns = Lr;  // the first sweep's last stage
+  const int she = pad_shift(sh);
+  const int pmask = sh ? (1 << sh) - 1 : 0;
+  const bool rstep = (Lr & pmask) == 0;
+  const int rstride = Lr + (sh ? (Lr >> sh) : 0);
+  cx<Rt> v[NB][RAD];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = jl + b * Tl;
+    if (active && j < Lr) {
+      // FP32 requests the spectrum factors first; FP64 loads each as it
+      // consumes it (register budget).  The stage twiddles of butterfly j
+      // (k = j, q = 0) are read as they are consumed.
+      constexpr bool kPre = sizeof(Rt) == 4;
+      cx<Rt> f[kPre ? RAD : 1];
+      if (kPre) {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) f[kPre ? t : 0] = gld(pm + (j + t * Lr) * pms);
+      }
+      if (rstep) {
+        const int p0 = pad_idx(j, she);
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) v[b][t] = line[p0 + t * rstride];
+      } else {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) v[b][t] = line[pad_idx(j + t * Lr, she)];
+      }
+      if (Ns > 1) {
+        const cx<Rt>* __restrict__ w = tw + (Ns - 1) + j * (RAD - 1);
+#pragma unroll
+        for (int t = 1; t < RAD; ++t) v[b][t] = cmul(v[b][t], gld(w + t - 1));
+      }
+      dft_any<Rt, RAD>(v[b]);
+      if (rader_dc >= 0 && j == 0) line[rader_dc] = v[b][0];
+#pragma unroll
+      for (int t = 0; t < RAD; ++t)
+        v[b][t] = cconj(cmul(v[b][t], kPre ? f[kPre ? t : 0] : gld(pm + (j + t * Lr) * pms)));
+      dft_any<Rt, RAD>(v[b]);
+    }
+  }
+  line_sync(ws);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = jl + b * Tl;
+    if (active && j < Lr) {
+      const int base = j * RAD;
+      if (sh == 0 || ((1 << sh) % RAD) == 0) {
+        // the RAD positions lie within one padding block (or no padding)
+        const int p0 = pad_idx(base, she);
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) line[p0 + t] = v[b][t];
+      } else {
+#pragma unroll
+        for (int t = 0; t < RAD; ++t) line[pad_idx(base + t, she)] = v[b][t];
+      }
+    }
+  }
+  line_sync(ws);
 }
 
 // All radix stages of a line.  With fuse_first the first stage reads its
 // inputs through ld(element) from global memory instead of LDS; with
-// fuse_last the last stage writes through st(element, value).
+// fuse_last the last stage writes through st(element, value).  Stages
+// [s_begin, s_end) run (default: all).
 template <class Rt, int EPT, bool FUSE, bool BIG, class LdF, class StF>
 __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const LinePass& p,
                                       const cx<Rt>* __restrict__ tw, const cx<Rt>* roots, bool fuse_first,
                                       bool fuse_last, const LdF& ld, const StF& st,
-                                      const cx<Rt>* __restrict__ pm = nullptr, int pms = 0) {
+                                      const cx<Rt>* __restrict__ pm = nullptr, int pms = 0, int s_begin = 0,
+                                      int s_end = -1) {
   int Ns = 1;
   const int sh = p.lds_sh;
-  const int nst = p.nst;
+  const bool ws = p.wave_sync != 0;
+  const int nst = s_end < 0 ? p.nst : s_end;
   // roots of the radix-P stages with P >= 17, one block of P per stage (LDS)
   int roots_off = 0;
-  for (int s = 0; s < nst; ++s) {
+  for (int s = 0; s < s_begin; ++s) {
+    if (BIG && big_stage(p.rad[s], kPrec<Rt>)) roots_off += p.rad[s];
+    Ns *= p.rad[s];
+  }
+  for (int s = s_begin; s < nst; ++s) {
     const int r = p.rad[s];
     const FastDiv nsd = p.nsdiv[s];
     const bool gl = fuse_first && s == 0;
-    const bool gs = fuse_last && s == nst - 1;
-    if (BIG && r >= 17) {
+    const bool gs = fuse_last && s == p.nst - 1;
+    if (BIG && big_stage(r, kPrec<Rt>)) {
       // the planner keeps fused global access and the in-stage spectrum
       // product off stages of this kind
-      stage_big<Rt>(line, jl, active, p.Tl, p.L, r, Ns, nsd, tw, roots + roots_off, sh);
+      stage_big<Rt>(line, jl, active, p.Tl, p.L, r, Ns, nsd, tw, roots + roots_off, sh, ws);
       roots_off += r;
       Ns *= r;
       continue;
     }
     switch (r) {
 #define FFTAHIP_STAGE(R) \
-  case R: stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == 0 ? pm : nullptr, pms); break;
+  case R: stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == s_begin ? pm : nullptr, pms, ws); break;
       FFTAHIP_STAGE(2)
       FFTAHIP_STAGE(3)
       FFTAHIP_STAGE(4)
@@ -338,6 +444,16 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
       FFTAHIP_STAGE(13)
       FFTAHIP_STAGE(16)
 #undef FFTAHIP_STAGE
+      // prime codelets: direct-DFT variants only (their registers)
+#define FFTAHIP_PSTAGE(R) \
+  case R: \
+    if constexpr (BIG && kPrec<Rt> == 32) \
+      stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == s_begin ? pm : nullptr, pms, ws); \
+    break;
+      FFTAHIP_PSTAGE(17)
+      FFTAHIP_PSTAGE(19)
+      FFTAHIP_PSTAGE(23)
+#undef FFTAHIP_PSTAGE
       default: break;
     }
     Ns *= r;
@@ -485,7 +601,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     roots = reinterpret_cast<cx<Rt>*>(reinterpret_cast<unsigned char*>(lds) + tile_lds_bytes(C, ldl, sizeof(cx<Rt>)));
     int nroots = 0;
     for (int s = 0; s < p.nst; ++s)
-      if (p.rad[s] >= 17) nroots += p.rad[s];
+      if (big_stage(p.rad[s], kPrec<Rt>)) nroots += p.rad[s];
     const cx<Rt>* __restrict__ src = reinterpret_cast<const cx<Rt>*>(p.tw) + (L - 1);
     for (int i = tid; i < nroots; i += BLOCK) roots[i] = gld(src + i);
   }
@@ -715,17 +831,50 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   };
   // only the plain variant carries the fused global paths (register cost)
   constexpr bool kFuse = MODE == MODE_PLAIN;
-  sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, fuse_in, fuse_out && !has_mid, ld, st);
-  if (rader) {
+  // FP32 only: in FP64 the fused stage's values and factors exceed the full
+  // variant's register budget (the planner sets mid_fused for FP32 passes)
+  constexpr bool kMidFuse = sizeof(Rt) == 4;
+  const bool mid_fused = kMidFuse && has_mid && p.mid_fused;
+  // wave-local sweeps (line_sync): the phases that map threads to lines
+  // differently (the separate spectrum phase, the store) need the workgroup
+  // barrier the sweeps' last stage no longer gives
+  const bool ws = p.wave_sync != 0;
+  sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, fuse_in, fuse_out && !has_mid, ld, st, nullptr, 0, 0,
+                             mid_fused ? p.nst - 1 : p.nst);
+  if (rader && !mid_fused) {
     // the DC term of the first sweep (sum of elements 1..P-1) before the
     // spectrum product overwrites it
     if (active && jl == 0) myline[ldl - kRaderSide + 1] = myline[0];
-    __syncthreads();
+    line_sync(ws);
   }
 
-  if (has_mid && p.mid_in_stage) {
+  if (kMidFuse && has_mid && p.mid_fused) {
+    // first sweep's last stage, spectrum product and second sweep's first
+    // stage in one LDS round trip (the planner sets mid_fused only when the
+    // schedule's first and last radix are the same codelet radix)
+    const int rdc = rader ? ldl - kRaderSide + 1 : -1;
+    const cx<Rt>* __restrict__ pm = bhat + sh.cmid[cs];
+    switch (p.rad[p.nst - 1]) {
+#define FFTAHIP_MIDSTAGE(R) \
+  case R: stage_mid<Rt, R, EPT>(myline, jl, active, p.Tl, p.L, tw, p.lds_sh, pm, (int)p.cme, rdc, ws); break;
+      FFTAHIP_MIDSTAGE(2)
+      FFTAHIP_MIDSTAGE(3)
+      FFTAHIP_MIDSTAGE(4)
+      FFTAHIP_MIDSTAGE(5)
+      FFTAHIP_MIDSTAGE(7)
+      FFTAHIP_MIDSTAGE(8)
+      FFTAHIP_MIDSTAGE(9)
+      FFTAHIP_MIDSTAGE(11)
+      FFTAHIP_MIDSTAGE(13)
+      FFTAHIP_MIDSTAGE(16)
+#undef FFTAHIP_MIDSTAGE
+      default: break;
+    }
+    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, nullptr, 0, 1);
+  } else if (has_mid && p.mid_in_stage) {
     sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, bhat + sh.cmid[cs], (int)p.cme);
   } else if (has_mid) {
+    if (ws) __syncthreads();
     if (p.row_mid && col_step) {
       if (row_c < nvalid) {
         const int c = row_c;
@@ -774,6 +923,8 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     __syncthreads();
     sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st);
   }
+
+  if (ws) __syncthreads();
 
   // ---- store (LDS -> global).  As on the load side, the table factors of a
   // batch are all requested before the first store of the batch.

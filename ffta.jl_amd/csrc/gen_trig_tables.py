@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Writes trig_tables.h: cos/sin(2 pi m / N) for the in-register DFT codelets.
 
-Every N the codelets take twiddles from gets a table: the radices 3..16 and
+Every N the codelets take twiddles from gets a table: the radices 3..16, the
+prime codelets 17, 19, 23 and
 every 13-smooth composite N <= 64 (the two-factor splits of the register
 line passes).  Values come from mpmath at 40 digits (cospi/sinpi, so that the
 exact points 0, +-1/2, +-1 are exact) and are printed with 21 significant
@@ -26,7 +27,7 @@ def fmt(x):
 
 
 def table_sizes():
-    base = [3, 5, 7, 8, 9, 11, 13, 16]
+    base = [3, 5, 7, 8, 9, 11, 13, 16, 17, 19, 23]
     comp = [n for n in range(6, 65) if smooth13(n) and n not in base and n not in (2, 4)]
     return sorted(set(base + comp))
 

@@ -70,6 +70,8 @@ hipError_t launch_real_boundary(int op, int precision, void* dst, const void* sr
 hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_t N, hipStream_t stream);
 // Chirp steps d[t] = exp(i pi (t^2 s^2 mod 2N) / N), t < L (FP64), of a register pass.
 hipError_t launch_gen_chirp_steps(double2* d, int64_t s, int64_t N, int L, hipStream_t stream);
+hipError_t launch_gen_chirp_range(void* out, int precision, int64_t n0, int64_t count, int64_t N,
+                                  hipStream_t stream);
 hipError_t launch_gen_chirp_ext(double2* bext, const double2* chirp64, int64_t N, int64_t M, hipStream_t stream);
 // Rader's convolution kernel b_r = exp(-2 pi i g^r / P), r < P - 1, FP64.
 hipError_t launch_gen_rader_b(double2* b, int64_t P, int64_t g, hipStream_t stream);

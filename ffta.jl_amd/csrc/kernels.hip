@@ -43,7 +43,7 @@ __global__ void gen_stage_twiddles_kernel(cx<Rt>* tw, RadixList rl, int64_t n) {
   int64_t off = Ns - 1;  // Ns = L here
   for (int s = 0; s < rl.nst; ++s) {
     const int64_t R = rl.rad[s];
-    if (R < 17) continue;
+    if (!big_stage((int)R, sizeof(Rt) == 4 ? 32 : 64)) continue;
     if (m >= off && m < off + R) {
       tw[m] = from_d2<Rt>(unit_root(m - off, R));
       return;
@@ -221,18 +221,35 @@ __global__ void brfft_split_kernel(cx<Rt>* A, const cx<Rt>* X, RealGeom g, int64
 }
 
 // Forward Bluestein chirp b_n = exp(+i pi p / N), p = n^2 mod 2N taken in
-// (-N, N] (src/algos.jl:353-359 with sgn = +1), FP64 evaluated.
+// (-N, N] (src/algos.jl:353-359 with sgn = +1), FP64 evaluated (exact
+// reduction: n < N < 2^31, so n^2 < 2^62).
+__device__ __forceinline__ double2 chirp_value(int64_t n, int64_t N) {
+  const int64_t twoN = 2 * N;
+  int64_t p = (int64_t)(((uint64_t)n * (uint64_t)n) % (uint64_t)twoN);
+  if (p > N) p -= twoN;
+  double s, c;
+  sincospi((double)p / (double)N, &s, &c);
+  return make_double2(c, s);
+}
+
+// The table of the plan precision (and its FP64 copy for the spectrum).
 template <class Rt>
 __global__ void gen_chirp_kernel(cx<Rt>* chirp, double2* chirp64, int64_t N) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
-  const int64_t twoN = 2 * N;
-  int64_t p = (int64_t)(((uint64_t)n * (uint64_t)n) % (uint64_t)twoN);  // exact: N < 2^31
-  if (p > N) p -= twoN;
-  double s, c;
-  sincospi((double)p / (double)N, &s, &c);
-  chirp[n] = {(Rt)c, (Rt)s};
-  if (chirp64) chirp64[n] = make_double2(c, s);
+  const double2 v = chirp_value(n, N);
+  chirp[n] = {(Rt)v.x, (Rt)v.y};
+  if (chirp64) chirp64[n] = v;
+}
+
+// Entries n0 .. n0 + count - 1 of the chirp of length N (read-back of the
+// table values at any N without building the whole table).
+template <class Rt>
+__global__ void gen_chirp_range_kernel(cx<Rt>* out, int64_t n0, int64_t count, int64_t N) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const double2 v = chirp_value(n0 + i, N);
+  out[i] = {(Rt)v.x, (Rt)v.y};
 }
 
 // Chirp steps of a register pass: d[t] = exp(i pi (t^2 s^2 mod 2N) / N),
@@ -452,6 +469,18 @@ hipError_t launch_gen_chirp(void* chirp, double2* chirp64, int precision, int64_
   else
     hipLaunchKernelGGL(gen_chirp_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
                        reinterpret_cast<cx<double>*>(chirp), chirp64, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_chirp_range(void* out, int precision, int64_t n0, int64_t count, int64_t N,
+                                  hipStream_t stream) {
+  const int64_t blocks = (count + 255) / 256;
+  if (precision == 32)
+    hipLaunchKernelGGL(gen_chirp_range_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<cx<float>*>(out), n0, count, N);
+  else
+    hipLaunchKernelGGL(gen_chirp_range_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<cx<double>*>(out), n0, count, N);
   return hipGetLastError();
 }
 

@@ -212,16 +212,22 @@ struct Inner {
   std::vector<int> rad;
   int Tl = 1;
   int block = 256;
+  // strided tiles of this pass (0: the default rule): LDS budget per tile in
+  // bytes and the fewest adjacent lines a tile keeps
+  int64_t tile_budget = 0, tile_cmin = 0;
 };
 
-// Radices of the in-register codelets (codelets.h); any other radix is a
-// prime P >= 17 run by the direct-DFT stage (stage_big in device.h).
-static bool codelet_radix(int r) {
+// Radices of the in-register codelets (codelets.h): 2..16 in every kernel
+// variant, the primes 17, 19, 23 in the FP32 direct-DFT variants; any other
+// radix is a prime P >= 17 run by the direct-DFT stage (stage_big in
+// device.h).  prec 0 (a length checked for any precision) counts as FP64.
+static bool small_radix(int r) {
   switch (r) {
     case 2: case 3: case 4: case 5: case 7: case 8: case 9: case 11: case 13: case 16: return true;
     default: return false;
   }
 }
+static bool codelet_radix(int r, int prec) { return small_radix(r) || prime_codelet(r, prec); }
 
 // Largest prime the direct-DFT stage takes (larger primes go through a
 // convolution whatever BLUESTEIN_CUTOFF says), and the most roots one
@@ -237,12 +243,12 @@ constexpr int64_t kBigKPHost = 4, kBigHoldHost = 2;
 // line Tl = max over stages of ceil((L / R) / (16 / R)) for codelet radices,
 // and for a direct-DFT stage of prime P the work items over kBigHold:
 // ceil((L / P) * ceil(((P - 1) / 2) / kBigKP) / kBigHold).
-static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in) {
+static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in, int prec) {
   if ((int)rad.size() > kMaxStages) return false;
   int64_t prod = 1, Tl = 1;
   for (int r : rad) {
-    if (codelet_radix(r)) {
-      const int64_t nb = kEPT / r;
+    if (codelet_radix(r, prec)) {
+      const int64_t nb = std::max<int64_t>(1, kEPT / r);
       Tl = std::max(Tl, ((L / r) + nb - 1) / nb);
     } else {
       if (r < 17 || r > kMaxDirectPrime || !is_prime(r)) return false;
@@ -253,7 +259,7 @@ static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in) {
   }
   int64_t roots = 0;
   for (int r : rad)
-    if (!codelet_radix(r)) roots += r;
+    if (!codelet_radix(r, prec)) roots += r;
   if (prod != L || Tl > 1024 || roots > kMaxDirectRoots) return false;
   in = Inner();
   in.L = L;
@@ -266,6 +272,25 @@ static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in) {
 // Radix schedules chosen by the autotuner, per (precision, line length).
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, int64_t>, std::vector<int>> g_tuned;
+
+// Strided tile shape chosen by the autotuner for the passes of a transform
+// length whose lines are laid across memory (multi-pass Stockham passes and
+// strided region dimensions): LDS bytes per tile and the fewest adjacent
+// lines a tile keeps.  A larger budget lets a long strided line run in one
+// pass with few lines per tile (e.g. 4 lines of 4096: one pass instead of
+// two of 64 x 64, one workgroup per CU); the default keeps 40 KB tiles of
+// >= 128-byte segments, several workgroups per CU.  Per (precision, length).
+struct TileShape {
+  int64_t budget = 0;  // 0: the default rule
+  int64_t cmin = 0;
+};
+static std::map<std::pair<int, int64_t>, TileShape> g_tuned_tile;
+
+static TileShape tuned_tile(int prec, int64_t N) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tuned_tile.find({prec, N});
+  return it == g_tuned_tile.end() ? TileShape() : it->second;
+}
 
 // Primes a length may contain and still run as direct DFT stages: 2 and 3
 // always (FFTA classifies powers of 2 and 3 before primes), any other prime
@@ -299,7 +324,7 @@ static bool make_inner(int64_t L, Inner& in, int prec = 0, int64_t cutoff = kInt
   if (prec) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tuned.find({prec, L});
-    if (it != g_tuned.end() && inner_from(L, it->second, in)) return true;
+    if (it != g_tuned.end() && inner_from(L, it->second, in, prec)) return true;
   }
   int64_t m = L;
   int e2 = 0, e3 = 0;
@@ -323,7 +348,7 @@ static bool make_inner(int64_t L, Inner& in, int prec = 0, int64_t cutoff = kInt
   while (e3 >= 2) { rad.push_back(9); e3 -= 2; }
   if (e3) rad.push_back(3);
   std::sort(rad.begin(), rad.end(), [](int a, int b) { return a > b; });
-  return inner_from(L, rad, in);
+  return inner_from(L, rad, in, prec);
 }
 
 // Candidate radix schedules of the autotuner for a line of length L: the
@@ -377,7 +402,7 @@ static std::vector<std::vector<int>> inner_candidates(int64_t L) {
         std::vector<int> c = r;
         if (order) std::reverse(c.begin(), c.end());
         Inner tmp;
-        if (inner_from(L, c, tmp) && seen.insert(c).second) out.push_back(c);
+        if (inner_from(L, c, tmp, 0) && seen.insert(c).second) out.push_back(c);
       }
     }
   return out;
@@ -463,7 +488,7 @@ static fftahip_status stage_table(Tables& T, const Inner& in, const void** out, 
     // L - 1 stage twiddles, then the P roots of each direct-DFT stage
     int64_t n = in.L - 1;
     for (int r : in.rad)
-      if (!codelet_radix(r)) n += r;
+      if (!codelet_radix(r, T.prec)) n += r;
     n = std::max<int64_t>(1, n);
     fftahip_status st = buf->ensure((size_t)n * csize(T.prec));
     if (st) return st;
@@ -555,7 +580,12 @@ static int64_t single_max(int prec) {
   int64_t m = lds_budget() / (int64_t)csize(prec) * 16 / 17;
   return std::min<int64_t>(m, 1024 * kEPT);
 }
-static int64_t pass_max(int prec) {
+static int64_t pass_max(int prec, const TileShape& ts = TileShape()) {
+  if (ts.budget > 0) {
+    // the longest line whose padded pitch (<= 17/16 L + 1) fits cmin lines
+    const int64_t b = std::min(ts.budget, lds_budget());
+    return std::max<int64_t>(16, (b / ((int64_t)csize(prec) * ts.cmin) - 1) * 16 / 17);
+  }
   const int64_t dflt = std::min<int64_t>(1024, pass_budget() / ((int64_t)csize(prec) * min_lines(prec)));
   const int64_t v = env_i64("FFTAHIP_PASS_MAX", dflt);
   return std::max<int64_t>(v, 16);
@@ -626,9 +656,63 @@ static bool split_factors(int64_t M, int64_t cap, std::vector<int64_t>& F, int64
   return false;
 }
 
+static fftahip_status make_decomp_rule(LengthPlan& lp, int strided, Decomp& d);
+
+// Strided passes of a decomposition carry the length's tuned tile shape.
+static void apply_tile(Decomp& d, const TileShape& ts) {
+  for (Inner& in : d.inner) {
+    in.tile_budget = ts.budget;
+    in.tile_cmin = ts.cmin;
+  }
+}
+
+// FP32 convolution lines (Bluestein, Rader, the three-pass row) run their
+// two sweeps with the boundary fused into one stage when the schedule starts
+// and ends with the same codelet radix: a radix that occurs at least twice
+// (the largest such) is moved to both ends, the others keep their order.
+static void mid_ends(Inner& in, int prec) {
+  if (prec != 32 || env_i64("FFTAHIP_MID_FUSE", 1) == 0 || in.rad.size() < 2) return;
+  if (in.rad.front() == in.rad.back() && small_radix(in.rad.front())) return;
+  if (!small_radix(in.rad.front())) return;  // a direct-DFT first stage keeps the product in its own phase
+  int pick = 0;
+  for (int r : in.rad)
+    if (small_radix(r) && std::count(in.rad.begin(), in.rad.end(), r) >= 2) pick = std::max(pick, r);
+  if (!pick) return;
+  std::vector<int> rad{pick};
+  bool skipped_first = false, skipped_second = false;
+  for (int r : in.rad) {
+    if (r == pick && !skipped_first) { skipped_first = true; continue; }
+    if (r == pick && !skipped_second) { skipped_second = true; continue; }
+    rad.push_back(r);
+  }
+  rad.push_back(pick);
+  Inner tmp;
+  if (!inner_from(in.L, rad, tmp, prec)) return;
+  tmp.tile_budget = in.tile_budget;
+  tmp.tile_cmin = in.tile_cmin;
+  in = tmp;
+}
+
 static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
+  fftahip_status st = make_decomp_rule(lp, strided, d);
+  if (st) return st;
+  apply_tile(d, tuned_tile(lp.prec, lp.N));
+  switch (d.kind) {
+    case K_BLUE_SINGLE: case K_RADER_SINGLE: case K_MIXED: case K_RADER_MIXED:
+      mid_ends(d.inner[0], lp.prec);
+      break;
+    case K_BLUE_3PASS:
+      mid_ends(d.inner[1], lp.prec);
+      break;
+    default: break;
+  }
+  return st;
+}
+
+static fftahip_status make_decomp_rule(LengthPlan& lp, int strided, Decomp& d) {
   d = Decomp();
   const int prec = lp.prec;
+  const TileShape ts = tuned_tile(prec, lp.N);
   if (lp.mixed) {
     // pass 0: Q-point DFTs as Bluestein lines of M in LDS (or Rader lines of
     // Q - 1); then Stockham over S
@@ -636,8 +720,8 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
     d.M = lp.N;
     std::vector<int64_t> FS;
     Inner tmp;
-    if (lp.S <= pass_max(prec) && make_inner(lp.S, tmp, prec, lp.cutoff)) FS = {lp.S};
-    else if (!split_factors(lp.S, pass_max(prec), FS, lp.cutoff))
+    if (lp.S <= pass_max(prec, ts) && make_inner(lp.S, tmp, prec, lp.cutoff)) FS = {lp.S};
+    else if (!split_factors(lp.S, pass_max(prec, ts), FS, lp.cutoff))
       return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(lp.S));
     d.F = {lp.bq};
     Inner qb;
@@ -665,7 +749,7 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
     d.inner = {in};
     return FFTAHIP_SUCCESS;
   }
-  const int64_t single_cap = strided ? pass_max(prec) : single_max(prec);
+  const int64_t single_cap = strided ? pass_max(prec, ts) : single_max(prec);
   if (target <= single_cap) {
     Inner in;
     if (make_inner(target, in, prec, cut)) {
@@ -686,7 +770,7 @@ static fftahip_status make_decomp(LengthPlan& lp, int strided, Decomp& d) {
     return FFTAHIP_SUCCESS;
   }
   std::vector<int64_t> F;
-  if (!split_factors(target, pass_max(prec), F, cut))
+  if (!split_factors(target, pass_max(prec, ts), F, cut))
     return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no pass decomposition for length " + std::to_string(target));
   d.kind = lp.blue ? K_BLUE_MULTI : K_MULTI;
   d.F = F;
@@ -882,30 +966,45 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
       p.nsdiv[s] = make_fastdiv(1);
     }
   }
-  p.Tl = in.Tl;
+  // threads per line: lines of up to 64 threads get a power of two (at most
+  // a third of the lanes idle), so that every line lies within one wavefront
+  // and the sweeps synchronise per wave (wave_sync) instead of per workgroup
+  int64_t Tl = in.Tl;
+  bool wave = false;
+  if (env_i64("FFTAHIP_WAVE_SYNC", 1) != 0 && Tl <= 64) {
+    int64_t t2 = 1;
+    while (t2 < Tl) t2 <<= 1;
+    if (t2 * 3 <= Tl * 4) {
+      Tl = t2;
+      wave = true;
+    }
+  }
+  p.Tl = (int32_t)Tl;
+  p.wave_sync = wave ? 1 : 0;
   const int64_t esz = (int64_t)csize(prec);
   const bool strided = p.col_in || p.col_out;
   // lines per workgroup: fill 256 threads; strided tiles also want
   // min_lines adjacent lines (128-byte segments) within the pass budget
-  int64_t C = std::max<int64_t>(1, 256 / in.Tl);
-  if (strided) C = std::max<int64_t>(C, min_lines(prec));
+  int64_t C = std::max<int64_t>(1, 256 / Tl);
+  if (strided) C = std::max<int64_t>(C, in.tile_cmin > 0 ? in.tile_cmin : min_lines(prec));
   C = std::min<int64_t>(C, kMaxLines);
-  C = std::min<int64_t>(C, std::max<int64_t>(1, 1024 / in.Tl));
+  C = std::min<int64_t>(C, std::max<int64_t>(1, 1024 / Tl));
   // fill the workgroup size the tile needs with as many lines as it holds
   // (e.g. 40 threads per line: 25 lines in 1024 threads, not 16 in 640)
   if (env_i64("FFTAHIP_FILL", 1) != 0) {
-    const int64_t thr = C * in.Tl;
+    const int64_t thr = C * Tl;
     const int64_t blk = thr <= 256 ? 256 : (thr <= 512 ? 512 : 1024);
-    C = std::max<int64_t>(C, std::min<int64_t>(kMaxLines, blk / in.Tl));
+    C = std::max<int64_t>(C, std::min<int64_t>(kMaxLines, blk / Tl));
   }
   // Rader lines keep x0 and the first sweep's DC term after their positions
   const int64_t side = p.load_mode == LOAD_RADER ? kRaderSide : 0;
   const int64_t padmax = in.L + (in.L - 1) / 16 + 1 + side;  // largest pitch any padding choice needs
-  const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pass_budget(), padmax * esz)) : lds_budget();
+  const int64_t pbud = in.tile_budget > 0 ? std::min(in.tile_budget, lds_budget()) : pass_budget();
+  const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pbud, padmax * esz)) : lds_budget();
   // direct-DFT stages stage their roots after the tile's line tables
   int64_t roots_bytes = 0;
   for (int r : in.rad)
-    if (!codelet_radix(r)) roots_bytes += r * esz;
+    if (!codelet_radix(r, prec)) roots_bytes += r * esz;
   while (C > 1 && C * padmax * esz + roots_bytes > budget) --C;
   if (C > p.nlines) C = std::max<int64_t>(1, p.nlines);
   // LDS padding: env FFTAHIP_LDS_PAD (-1 = choose by the bank model)
@@ -916,7 +1015,9 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
   const int words = (int)(esz / 4);
   static std::mutex pad_mu;
   static std::map<std::vector<int64_t>, std::pair<int, int64_t>> pad_cache;
-  std::vector<int64_t> key = {in.L, in.Tl, C, words, p.col_in, p.col_out, force, side};
+  Inner inw = in;  // the line schedule with the threads per line this pass runs
+  inw.Tl = (int)Tl;
+  std::vector<int64_t> key = {in.L, Tl, C, words, p.col_in, p.col_out, force, side};
   for (int r : in.rad) key.push_back(r);
   {
     std::lock_guard<std::mutex> g(pad_mu);
@@ -933,7 +1034,7 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
     for (int extra : {0, 1}) {
       if (C == 1 && extra) continue;
       const int64_t pitch = padded((int)in.L - 1, sh) + 1 + extra + side;
-      const double cst = lds_cost(in, (int)C, (int)pitch, sh, words, p.col_in, p.col_out);
+      const double cst = lds_cost(inw, (int)C, (int)pitch, sh, words, p.col_in, p.col_out);
       if (cst < best - 1e-9) {
         best = cst;
         best_sh = sh;
@@ -954,13 +1055,13 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
   // when the threads of one line cover >= 128 contiguous bytes per access
   // and the pass has no table-driven element operations
   const bool plain = p.load_mode == LOAD_PLAIN && p.mid_mode == MID_NONE && p.store_mode == STORE_PLAIN && p.pretw == 0;
-  const bool fuse = env_i64("FFTAHIP_FUSE", 1) != 0 && !in.rad.empty() && plain && in.Tl * esz >= 128;
+  const bool fuse = env_i64("FFTAHIP_FUSE", 1) != 0 && !in.rad.empty() && plain && Tl * esz >= 128;
   // the direct-DFT stage reads and writes LDS only
-  const bool big_first = !in.rad.empty() && !codelet_radix(in.rad.front());
-  const bool big_last = !in.rad.empty() && !codelet_radix(in.rad.back());
+  const bool big_first = !in.rad.empty() && !codelet_radix(in.rad.front(), prec);
+  const bool big_last = !in.rad.empty() && !codelet_radix(in.rad.back(), prec);
   p.fuse_in = (fuse && !p.col_in && !big_first) ? 1 : 0;
   p.fuse_out = (fuse && !p.col_out && !big_last) ? 1 : 0;
-  const int64_t threads = C * in.Tl;
+  const int64_t threads = C * Tl;
   p.C = (int32_t)C;
   p.ldl = (int32_t)ldl;
   p.divC = make_fastdiv((uint32_t)C);
@@ -981,6 +1082,10 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
       p.row_out = p.col_out ? 0 : 1;
     }
     p.mid_in_stage = (p.mid_mode == MID_BHAT && env_i64("FFTAHIP_MID_STAGE", 1) != 0 && !big_first) ? 1 : 0;
+    p.mid_fused = (p.mid_in_stage && prec == 32 && env_i64("FFTAHIP_MID_FUSE", 1) != 0 && !in.rad.empty() &&
+                   in.rad.front() == in.rad.back() && small_radix(in.rad.back()))
+                      ? 1
+                      : 0;
   }
   L.lds = tile_lds_bytes(C, ldl, esz) + (size_t)roots_bytes;
   L.grid = (p.nlines + C - 1) / C;
@@ -1529,7 +1634,7 @@ static int64_t best_line_length(int64_t n, int prec) {
 static double inner_work(const Inner& in) {
   const double div = (double)std::max<int64_t>(1, env_i64("FFTAHIP_BIG_DIV", 8));
   double w = 0;
-  for (int r : in.rad) w += codelet_radix(r) ? 1.0 : r / div;
+  for (int r : in.rad) w += small_radix(r) ? 1.0 : r / div;
   return w;
 }
 
@@ -2902,7 +3007,7 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   os << ", ";
   json_ptr(os, "tw_hi", p.tw_hi);
   os << ", \"conj_in\": " << p.conj_in << ", \"conj_out\": " << p.conj_out << ", \"scale\": " << p.scale
-     << ", \"load_mode\": " << p.load_mode << ", \"mid_mode\": " << p.mid_mode << ", \"store_mode\": " << p.store_mode
+     << ", \"load_mode\": " << p.load_mode << ", \"mid_mode\": " << p.mid_mode << ", \"mid_in_stage\": " << p.mid_in_stage << ", \"mid_fused\": " << p.mid_fused << ", \"wave_sync\": " << p.wave_sync << ", \"store_mode\": " << p.store_mode
      << ", \"xcd_remap\": " << p.xcd_remap << ", \"chirp_n\": " << p.chirp_n << ", \"ca\": [" << p.ca[0] << ", "
      << p.ca[1] << ", " << p.ca[2] << ", " << p.ca[3] << "], \"cb\": [" << p.cb[0] << ", " << p.cb[1] << ", "
      << p.cb[2] << ", " << p.cb[3] << "], \"cm\": [" << p.cm[0] << ", " << p.cm[1] << ", " << p.cm[2] << ", "
@@ -3057,17 +3162,111 @@ fftahip_status fftahip_autotune_lines(int precision, int64_t count, const int64_
   return st;
 }
 
+// Strided tile shapes the shape autotuner tries (LDS bytes per tile, fewest
+// adjacent lines): the default rule first, then larger tiles with fewer
+// lines, which let longer strided lines run in one pass.
+static const TileShape kTileCands[] = {{0, 0}, {71680, 8}, {71680, 4}, {143360, 8}, {143360, 4}, {143360, 2}};
+
+fftahip_status fftahip_autotune_shape(int precision, int ndims, const int64_t* dims, int nregion,
+                                      const int64_t* region, int64_t* changed_out) {
+  if (precision != 32 && precision != 64) return fail(FFTAHIP_ERR_ARGUMENT, "precision must be 32 or 64");
+  if (ndims < 1 || !dims || nregion < 1 || !region) return fail(FFTAHIP_ERR_ARGUMENT, "NULL or empty shape");
+  int64_t total = 1;
+  for (int i = 0; i < ndims; ++i) {
+    if (dims[i] <= 0) return fail(FFTAHIP_ERR_DIMENSION_MISMATCH, "Array length must be strictly positive");
+    total *= dims[i];
+  }
+  // the transform lengths of the region (one tile shape for all of them)
+  std::set<int64_t> lens;
+  for (int k = 0; k < nregion; ++k) {
+    if (region[k] < 1 || region[k] > ndims) return fail(FFTAHIP_ERR_ARGUMENT, "Plan region is outside array dimensions.");
+    lens.insert(dims[region[k] - 1]);
+  }
+  const size_t bytes = (size_t)total * csize(precision);
+  DevBuf a, b;
+  fftahip_status st;
+  if ((st = a.ensure(bytes))) return st;
+  if ((st = b.ensure(bytes))) return st;
+  HIP_TRY(hipMemset(a.p, 0, bytes));
+  HIP_TRY(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  auto set_all = [&](const TileShape& ts) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (int64_t N : lens) {
+      if (ts.budget > 0) g_tuned_tile[{precision, N}] = ts;
+      else g_tuned_tile.erase({precision, N});
+    }
+  };
+  double best_ms = 1e30;
+  TileShape best;
+  std::set<std::string> seen;  // candidates giving the same launches are timed once
+  for (const TileShape& c : kTileCands) {
+    set_all(c);
+    fftahip_plan pl = nullptr;
+    if ((st = fftahip_plan_create(&pl, precision, -1, ndims, dims, nregion, region, 73))) break;
+    std::string key;
+    {
+      std::vector<Launch> L;
+      size_t need = 0;
+      if ((st = plan_schedule(pl, ndims, dims, L, &need, nullptr))) {
+        fftahip_plan_destroy(pl);
+        break;
+      }
+      std::ostringstream os;
+      for (const Launch& la : L) os << la.grid << ',' << la.block << ',' << la.lds << ',' << la.lp.L << ';';
+      key = os.str();
+    }
+    if (!seen.insert(key).second) {
+      fftahip_plan_destroy(pl);
+      continue;
+    }
+    float ms = 0;
+    st = fftahip_execute(pl, ndims, dims, a.p, b.p, nullptr);
+    if (st == FFTAHIP_SUCCESS) {
+      hipError_t e = hipDeviceSynchronize();
+      int reps = 0;
+      if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+      for (; e == hipSuccess && reps < 10; ++reps)
+        if ((st = fftahip_execute(pl, ndims, dims, a.p, b.p, nullptr))) break;
+      if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess && st == FFTAHIP_SUCCESS) st = fail(FFTAHIP_ERR_HIP, hipGetErrorString(e));
+      ms /= std::max(reps, 1);
+    }
+    fftahip_plan_destroy(pl);
+    if (st) break;
+    // a larger tile must win by 3% (run-to-run noise) to replace the rule
+    if (ms < (c.budget > 0 ? 0.97 : 1.0) * best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  set_all(st ? TileShape() : best);
+  if (changed_out) *changed_out = (!st && best.budget > 0) ? (int64_t)lens.size() : 0;
+  return st;
+}
+
 fftahip_status fftahip_autotune_save(const char* path) {
   if (!path) return fail(FFTAHIP_ERR_ARGUMENT, "NULL path");
   FILE* f = std::fopen(path, "w");
   if (!f) return fail(FFTAHIP_ERR_ARGUMENT, std::string("cannot write ") + path);
-  std::fprintf(f, "# fftahip radix schedules: precision line-length radices (stage order)\n");
+  std::fprintf(f, "# fftahip tuning table\n");
+  std::fprintf(f, "# radix schedule:     precision line-length radices (stage order)\n");
+  std::fprintf(f, "# strided tile shape: S precision transform-length lds-bytes-per-tile fewest-lines\n");
   std::lock_guard<std::mutex> g(g_tune_mu);
   for (const auto& kv : g_tuned) {
     std::fprintf(f, "%d %lld", kv.first.first, (long long)kv.first.second);
     for (int r : kv.second) std::fprintf(f, " %d", r);
     std::fprintf(f, "\n");
   }
+  for (const auto& kv : g_tuned_tile)
+    std::fprintf(f, "S %d %lld %lld %lld\n", kv.first.first, (long long)kv.first.second, (long long)kv.second.budget,
+                 (long long)kv.second.cmin);
   std::fclose(f);
   return FFTAHIP_SUCCESS;
 }
@@ -3081,6 +3280,17 @@ fftahip_status fftahip_autotune_load(const char* path, int64_t* entries_out) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#') continue;
+    if (line[0] == 'S') {
+      std::istringstream is(line + 1);
+      int prec;
+      long long N, budget, cmin;
+      if (!(is >> prec >> N >> budget >> cmin)) continue;
+      if ((prec != 32 && prec != 64) || N < 1 || budget < 4096 || budget > (1 << 20) || cmin < 1 || cmin > kMaxLines)
+        continue;  // skip malformed entries
+      g_tuned_tile[{prec, (int64_t)N}] = TileShape{budget, cmin};
+      ++n;
+      continue;
+    }
     std::istringstream is(line);
     int prec;
     long long L;
@@ -3089,7 +3299,7 @@ fftahip_status fftahip_autotune_load(const char* path, int64_t* entries_out) {
     int r;
     while (is >> r) rad.push_back(r);
     Inner tmp;
-    if ((prec != 32 && prec != 64) || !inner_from(L, rad, tmp)) continue;  // skip malformed entries
+    if ((prec != 32 && prec != 64) || !inner_from(L, rad, tmp, prec)) continue;  // skip malformed entries
     g_tuned[{prec, (int64_t)L}] = rad;
     ++n;
   }
@@ -3101,6 +3311,15 @@ fftahip_status fftahip_autotune_load(const char* path, int64_t* entries_out) {
 fftahip_status fftahip_autotune_clear(void) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   g_tuned.clear();
+  g_tuned_tile.clear();
+  return FFTAHIP_SUCCESS;
+}
+
+fftahip_status fftahip_autotune_get_tile(int precision, int64_t length, int64_t* lds_bytes, int64_t* min_lines_out) {
+  if (!lds_bytes || !min_lines_out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  const TileShape ts = tuned_tile(precision, length);
+  *lds_bytes = ts.budget;
+  *min_lines_out = ts.cmin;
   return FFTAHIP_SUCCESS;
 }
 
@@ -3111,6 +3330,36 @@ fftahip_status fftahip_autotune_get(int precision, int64_t length, int* nrad, in
   *nrad = (int)in.rad.size();
   if (radices)
     for (size_t i = 0; i < in.rad.size(); ++i) radices[i] = in.rad[i];
+  return FFTAHIP_SUCCESS;
+}
+
+// Device read-back of the chirp values the plans use (the same device
+// function as the chirp tables): entries n0 .. n0 + count - 1 of length N in
+// the given precision.
+fftahip_status fftahip_debug_chirp(int precision, int64_t N, int64_t n0, int64_t count, void* host_out) {
+  if (precision != 32 && precision != 64) return fail(FFTAHIP_ERR_ARGUMENT, "precision must be 32 or 64");
+  if (N < 1 || N >= (int64_t(1) << 31) || n0 < 0 || count < 0 || n0 + count > N)
+    return fail(FFTAHIP_ERR_ARGUMENT, "need 0 <= n0 <= n0 + count <= N < 2^31");
+  if (count == 0) return FFTAHIP_SUCCESS;
+  if (!host_out) return fail(FFTAHIP_ERR_ARGUMENT, "NULL output");
+  DevBuf d;
+  fftahip_status st;
+  if ((st = d.ensure((size_t)count * csize(precision)))) return st;
+  HIP_TRY(launch_gen_chirp_range(d.p, precision, n0, count, N, nullptr));
+  HIP_TRY(hipMemcpy(host_out, d.p, (size_t)count * csize(precision), hipMemcpyDeviceToHost));
+  return FFTAHIP_SUCCESS;
+}
+
+// Device read-back of a register pass's FP64 chirp steps d[t] = exp(i pi
+// ((t s mod 2N)^2 mod 2N) / N), t < L.
+fftahip_status fftahip_debug_chirp_steps(int64_t N, int64_t s, int L, double* host_out) {
+  if (N < 1 || N >= (int64_t(1) << 31) || s < 0 || s >= 2 * N || L < 1 || L > 4096 || !host_out)
+    return fail(FFTAHIP_ERR_ARGUMENT, "need 1 <= N < 2^31, 0 <= s < 2N, 1 <= L <= 4096, output");
+  DevBuf d;
+  fftahip_status st;
+  if ((st = d.ensure((size_t)L * sizeof(double2)))) return st;
+  HIP_TRY(launch_gen_chirp_steps((double2*)d.p, s, N, L, nullptr));
+  HIP_TRY(hipMemcpy(host_out, d.p, (size_t)L * sizeof(double2), hipMemcpyDeviceToHost));
   return FFTAHIP_SUCCESS;
 }
 

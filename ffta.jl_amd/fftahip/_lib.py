@@ -72,6 +72,8 @@ SIGNATURES = {
     "fftahip_server_reset_stats": (_st, [_vp]),
     "fftahip_server_destroy": (_st, [_vp]),
     "fftahip_autotune_lines": (_st, [ctypes.c_int, ctypes.c_int64, _i64p, _i64p]),
+    "fftahip_autotune_shape": (_st, [ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int, _i64p, _i64p]),
+    "fftahip_autotune_get_tile": (_st, [ctypes.c_int, ctypes.c_int64, _i64p, _i64p]),
     "fftahip_autotune_save": (_st, [ctypes.c_char_p]),
     "fftahip_autotune_load": (_st, [ctypes.c_char_p, _i64p]),
     "fftahip_autotune_clear": (_st, []),
@@ -89,6 +91,9 @@ SIGNATURES = {
     "fftahip_debug_batch_schedule": (_st, [ctypes.c_int, ctypes.c_int64, _i64p, ctypes.POINTER(ctypes.c_int),
                                            ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_size_t)]),
+    "fftahip_debug_chirp": (_st, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
+    "fftahip_debug_chirp_steps": (_st, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]),
     "fftahip_debug_fastdiv": (_st, [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64]),
 }

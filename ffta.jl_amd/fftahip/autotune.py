@@ -1,10 +1,13 @@
-"""Autotuner of the in-LDS radix schedules (include/fftahip.h, autotuner).
+"""Autotuner of the in-LDS radix schedules and the strided tile shapes
+(include/fftahip.h, autotuner).
 
 FFTA picks radices by fixed rules (src/callgraph.jl:76-126); here the library
 times the candidate schedules of each line length on the device and keeps the
 fastest.  `tune_batch` tunes the line lengths a ragged batch runs, heaviest
-first; the resulting table can be saved and loaded (text, one line per
-(precision, length)), and applies to every plan created after it is set.
+first; `tune_shape` times the strided tile shapes (LDS bytes per workgroup,
+lines per tile) of one array shape.  The resulting table can be saved and
+loaded (text, one line per entry), and applies to every plan created after it
+is set.
 """
 from __future__ import annotations
 
@@ -44,6 +47,27 @@ def tune_batch(lengths, directions, precision: int = 32, top: int | None = None)
         ll = ll[:top]
     changed = tune_lines([L for L, _ in ll], precision)
     return {"line_lengths": len(ll), "changed": changed}
+
+
+def tune_shape(shape, precision: int = 32, region=None) -> int:
+    """Time the strided tile shapes for the plan of `shape` over `region`
+    (default: every dimension) and keep the fastest for its transform lengths;
+    returns how many lengths now use a non-default tile."""
+    dims = [int(v) for v in shape]
+    reg = list(range(1, len(dims) + 1)) if region is None else ([int(region)] if isinstance(region, int)
+                                                                  else [int(r) for r in region])
+    changed = ctypes.c_int64(0)
+    _check(load().fftahip_autotune_shape(precision, len(dims), i64_array(dims), len(reg), i64_array(reg),
+                                         ctypes.byref(changed)))
+    return changed.value
+
+
+def tile(precision: int, length: int) -> tuple[int, int]:
+    """(LDS bytes per tile, fewest lines) of the strided passes of a transform of
+    `length`; (0, 0) is the default rule."""
+    b, c = ctypes.c_int64(0), ctypes.c_int64(0)
+    _check(load().fftahip_autotune_get_tile(precision, int(length), ctypes.byref(b), ctypes.byref(c)))
+    return b.value, c.value
 
 
 def save(path: str) -> None:

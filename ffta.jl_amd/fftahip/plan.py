@@ -402,6 +402,21 @@ def debug_batch_schedule(lengths, directions, precision: int = 32, lds_per_block
     return json.loads(txt)
 
 
+def debug_chirp(N: int, precision: int = 64, n0: int = 0, count: int | None = None) -> np.ndarray:
+    """The device's chirp entries n0 .. n0 + count - 1 of length N (needs a GPU)."""
+    count = N - n0 if count is None else count
+    out = np.empty(count, dtype=np.complex64 if precision == 32 else np.complex128)
+    _check(load().fftahip_debug_chirp(precision, N, n0, count, out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def debug_chirp_steps(N: int, s: int, L: int) -> np.ndarray:
+    """A register pass's FP64 chirp steps d[t], t < L, computed on the device (needs a GPU)."""
+    out = np.empty(L, dtype=np.complex128)
+    _check(load().fftahip_debug_chirp_steps(N, s, L, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
 def debug_fastdiv(d: int, n: np.ndarray) -> np.ndarray:
     n = np.ascontiguousarray(n, dtype=np.uint32)
     q = np.empty_like(n)

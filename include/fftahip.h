@@ -228,13 +228,27 @@ fftahip_status fftahip_server_destroy(fftahip_server server);
  * on this device with a batch of lines of that length and keep the fastest
  * for every plan created afterwards in this process; *changed_out counts the
  * lengths whose choice differs from the default rule.  The table can be saved
- * and loaded (text: "precision length radices..."). */
+ * and loaded (text: "precision length radices..." per radix schedule,
+ * "S precision length lds-bytes fewest-lines" per strided tile shape). */
 fftahip_status fftahip_autotune_lines(int precision, int64_t count, const int64_t* lengths, int64_t* changed_out);
+/* Tile shape of the strided passes (lines laid across memory: multi-pass
+ * Stockham passes, strided region dimensions) for the transform lengths of
+ * one array shape: times the plan of `dims` over `region` (forward) with
+ * each candidate tile (LDS bytes per workgroup, fewest adjacent lines per
+ * tile; the default rule is 40 KB tiles of >= 16 (CF32) / 8 (CF64) lines)
+ * and keeps the fastest for those lengths in every later plan; a larger
+ * tile must win by 3%.  *changed_out counts the lengths set to a non-default
+ * shape.  Allocates two arrays of the shape on the device. */
+fftahip_status fftahip_autotune_shape(int precision, int ndims, const int64_t* dims, int nregion,
+                                      const int64_t* region, int64_t* changed_out);
 fftahip_status fftahip_autotune_save(const char* path);
 fftahip_status fftahip_autotune_load(const char* path, int64_t* entries_out);
 fftahip_status fftahip_autotune_clear(void);
 /* The radix schedule (stage order) plans use for an in-LDS line of `length`. */
 fftahip_status fftahip_autotune_get(int precision, int64_t length, int* nrad, int* radices);
+/* The strided tile shape plans use for a transform of `length` (0, 0: the
+ * default rule). */
+fftahip_status fftahip_autotune_get_tile(int precision, int64_t length, int64_t* lds_bytes, int64_t* min_lines);
 
 /* ---------------------------------------------------------------- device helpers
  * Minimal memory helpers so that a host language without a GPU array type
@@ -259,6 +273,16 @@ fftahip_status fftahip_debug_schedule(int precision, int direction, int ndims, c
 fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const int64_t* lengths,
                                             const int* directions, int64_t lds_per_block, char* buf, size_t cap,
                                             size_t* needed);
+/* Device read-back of the Bluestein chirp the plans use, b_n = exp(i pi p /
+ * N), p = n^2 mod 2N folded into (-N, N] (src/algos.jl:353-359; the index
+ * arithmetic is exact 64-bit integer work): entries n0 .. n0 + count - 1
+ * for a length N < 2^31, as interleaved complex values of `precision` (the
+ * FP64 value rounded once).  Needs a device. */
+fftahip_status fftahip_debug_chirp(int precision, int64_t N, int64_t n0, int64_t count, void* host_out);
+/* Device read-back of a register pass's FP64 chirp steps, d[t] = exp(i pi
+ * ((t s mod 2N)^2 mod 2N) / N) for t < L (interleaved re, im).  Needs a
+ * device. */
+fftahip_status fftahip_debug_chirp_steps(int64_t N, int64_t s, int L, double* host_out);
 /* Host evaluation of the kernels' division by a runtime constant d. */
 fftahip_status fftahip_debug_fastdiv(uint32_t d, const uint32_t* n, uint32_t* q, int64_t count);
 

@@ -18,7 +18,15 @@ TK_STAGE, TK_TWLO, TK_TWHI, TK_CHIRP, TK_BHAT, TK_IN, TK_OUT, TK_SCRATCH, TK_BHA
     TK_CHIRPD, TK_RTAB = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13
 LOAD_RADER, STORE_RADER, RADER_SIDE = 2, 3, 2
 EPT = 16
+# in-register codelets (codelets.h); the primes 17, 19 and 23 only in the
+# FP32 direct-DFT kernel variants, one butterfly per thread
 CODELETS = {2, 3, 4, 5, 7, 8, 9, 11, 13, 16}
+PRIME_CODELETS = {17, 19, 23}
+
+
+def codelets(esize: int) -> set:
+    """Radices that run as codelets in a pass of element size esize (8: FP32)."""
+    return CODELETS | PRIME_CODELETS if esize == 8 else CODELETS
 # direct-DFT stage of a prime radix P >= 17 (device.h stage_big): work items
 # of BIG_KP output pairs, BIG_HOLD per thread
 BIG_KP, BIG_HOLD = 4, 2
@@ -66,11 +74,11 @@ def rader_tables(P: int):
     return rtab, np.fft.fft(b) / L
 
 
-def stage_table(L: int, rad) -> np.ndarray:
+def stage_table(L: int, rad, cod=CODELETS) -> np.ndarray:
     """Per-stage twiddles as the device table holds them: stage s (Ns = product
     of earlier radices, radix R) stores w_{Ns*R}^{t*k}, k < Ns, t = 1..R-1, at
     (Ns-1) + k*(R-1) + t-1."""
-    extra = sum(R for R in rad if R not in CODELETS)
+    extra = sum(R for R in rad if R not in cod)
     tab = np.zeros(max(1, L - 1 + extra), dtype=np.complex128)
     Ns = 1
     for R in rad:
@@ -81,7 +89,7 @@ def stage_table(L: int, rad) -> np.ndarray:
     # then the P roots w_P^m of each direct-DFT stage
     off = L - 1
     for R in rad:
-        if R not in CODELETS:
+        if R not in cod:
             tab[off:off + R] = np.exp(-2j * np.pi * np.arange(R) / R)
             off += R
     return tab
@@ -132,13 +140,13 @@ def _pad(i, sh):
     return i + (i >> sh) if sh else i
 
 
-def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None) -> None:
+def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None, cod=CODELETS) -> None:
     """Replay the kernel's stage loop for one line with its exact LDS
     positions (padding, stepped addressing) and table indexing; raise if an
     output slot is missed or written twice, or if the result is not the DFT."""
     rng = rng or np.random.default_rng(0)
     x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
-    tw = stage_table(L, rad)
+    tw = stage_table(L, rad, cod)
     P = _pad(L - 1, sh) + 1
     lds = np.full(P, np.nan + 0j)
     lds[_pad(np.arange(L), sh)] = x
@@ -147,7 +155,7 @@ def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None) -> None:
     roots_off = L - 1
     for R in rad:
         Lr = L // R
-        if R not in CODELETS:
+        if R not in cod:
             new, hits = _big_stage(lds, tw, L, R, Ns, Tl, sh, roots_off, P)
             roots_off += R
             if not np.all(hits[_pad(np.arange(L), sh)] == 1) or hits.sum() != L:
@@ -155,12 +163,13 @@ def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None) -> None:
             lds = new
             Ns *= R
             continue
-        NB = EPT // R
+        NB = max(1, EPT // R)
         if NB < 1:
             raise ScheduleError(f"radix {R} exceeds {EPT} registers")
         if Tl * NB < Lr:
             raise ScheduleError(f"L={L} radix {R}: {Tl} threads x {NB} butterflies < {Lr}")
-        rstep, wstep = (Lr & pmask) == 0, (Ns & pmask) == 0
+        rstep = (Lr & pmask) == 0
+        wstep = (Ns & pmask) == 0 or (Ns == 1 and (pmask + 1) % R == 0)
         rstride = Lr + ((Lr >> sh) if sh else 0)
         wstride = Ns + ((Ns >> sh) if sh else 0)
         new = np.full(P, np.nan + 0j)
@@ -265,9 +274,12 @@ class Emulator:
         else:
             if C < 1 or C > 256 or C * Tl > p["block"]:
                 raise ScheduleError(f"bad tile C={C} Tl={Tl} block={p['block']}")
+            if p.get("wave_sync") and (Tl > 64 or 64 % Tl):
+                # wave-synchronised sweeps need every line within one wavefront
+                raise ScheduleError(f"wave_sync with {Tl} threads per line")
             sh = p["lds_sh"]
             need = L + ((L - 1) >> sh if sh else 0) + (RADER_SIDE if p["load_mode"] == LOAD_RADER else 0)
-            roots = sum(r for r in p["rad"] if r not in CODELETS) * self.esize
+            roots = sum(r for r in p["rad"] if r not in codelets(self.esize)) * self.esize
             if p["ldl"] < need or p["lds"] != ((C * p["ldl"] * self.esize + 15) & ~15) + C * LINE_TAB_BYTES + roots:
                 raise ScheduleError("LDS size mismatch")
             if p["lds"] > lds_per_block:
@@ -276,9 +288,9 @@ class Emulator:
                 raise ScheduleError("grid does not cover all lines")
             if p["xcd_remap"] and p["grid"] % 8:
                 raise ScheduleError("XCD remap needs grid % 8 == 0")
-            key = (L, tuple(p["rad"]), Tl, p["lds_sh"])
+            key = (L, tuple(p["rad"]), Tl, p["lds_sh"], self.esize)
             if key not in self.sweeps_checked:
-                emulate_sweep(L, p["rad"], Tl, p["lds_sh"])
+                emulate_sweep(L, p["rad"], Tl, p["lds_sh"], cod=codelets(self.esize))
                 self.sweeps_checked.add(key)
         if p["tw"] is None or p["tw"][0] != TK_STAGE or p["tw"][1] != L:
             raise ScheduleError("stage twiddle table does not match L")

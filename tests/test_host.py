@@ -350,6 +350,59 @@ def test_autotune_table(tmp_path):
         autotune.clear()
 
 
+def test_autotune_tile_table(tmp_path):
+    """Strided tile shapes in the tuning table ("S precision length bytes
+    lines"): a long strided line of a tuned length runs in one pass of few
+    lines per tile instead of two 40 KB passes, the schedule still transforms
+    exactly and stays in bounds (emulator), malformed entries are skipped and
+    save/load/clear round-trip them."""
+    from fftahip import autotune
+
+    autotune.clear()
+    try:
+        dflt = P.debug_schedule((8, 4096), 32, -1, 2)["launches"]
+        assert len(dflt) == 2 and all(la["L"] == 64 for la in dflt)
+        src = tmp_path / "t.txt"
+        src.write_text("S 32 4096 143360 4\nS 32 2187 71680 8\nS 64 12 100 4\nS 32 99 143360 0\nS 16 5 143360 4\n")
+        assert autotune.load_table(str(src)) == 2
+        assert autotune.tile(32, 4096) == (143360, 4)
+        assert autotune.tile(64, 4096) == (0, 0)
+        la = P.debug_schedule((8, 4096), 32, -1, 2)["launches"]
+        assert len(la) == 1 and la[0]["L"] == 4096 and la[0]["C"] == 4 and la[0]["col_in"] == 1
+        assert la[0]["lds"] <= 160 * 1024
+        _check_schedule((8, 4096), 32, -1, 2)
+        _check_schedule((8, 4096), 32, 1, None)
+        # the contiguous dimension keeps its one-pass schedule: same radices as the strided one
+        la2 = P.debug_schedule((4096, 4096), 32, -1)["launches"]
+        assert [x["L"] for x in la2] == [4096, 4096] and la2[0]["rad"] == la2[1]["rad"]
+        out = tmp_path / "o.txt"
+        autotune.save(str(out))
+        assert "S 32 4096 143360 4" in out.read_text()
+        autotune.clear()
+        assert autotune.tile(32, 4096) == (0, 0)
+        assert autotune.load_table(str(out)) == 2
+        assert autotune.tile(32, 2187) == (71680, 8)
+    finally:
+        autotune.clear()
+
+
+def test_debug_chirp_argument_checks():
+    """The chirp read-back entry points refuse bad arguments before touching a device."""
+    import ctypes
+
+    lib = _lib.load()
+    out = np.zeros(8, dtype=np.complex128)
+    ptr = out.ctypes.data_as(ctypes.c_void_p)
+    assert lib.fftahip_debug_chirp(16, 10, 0, 1, ptr) == 2
+    assert lib.fftahip_debug_chirp(64, 1 << 31, 0, 1, ptr) == 2
+    assert lib.fftahip_debug_chirp(64, 10, 8, 3, ptr) == 2
+    assert lib.fftahip_debug_chirp(64, 10, -1, 1, ptr) == 2
+    assert lib.fftahip_debug_chirp(64, 10, 0, 0, None) == 0
+    dptr = out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.fftahip_debug_chirp_steps(10, 20, 4, dptr) == 2
+    assert lib.fftahip_debug_chirp_steps(10, 3, 0, dptr) == 2
+
+
 def _chirp_direct(n: int, N: int) -> complex:
     # the chirp table's definition: exp(i pi r / N), r = n^2 mod 2N folded into (-N, N]
     r = (n * n) % (2 * N)
