@@ -54,6 +54,19 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, FastDiv f) {
   return (uint32_t)(((uint64_t)hi + n) >> f.s);
 }
 
+// m mod d for d < 2^32 and m < 2^62 whose quotient is below 2^31, exact:
+// the FP64 product m * (1/d) is within 2^-12 of m/d (m rounds to FP64 with an
+// error below 2^9 when m >= 2^53, and then d > 2^22; the two roundings add a
+// relative 2^-52), so the truncated quotient is off by at most one, which one
+// correction step removes.  About a third of a generic 64-bit remainder.
+__host__ __device__ inline uint64_t mod_q31(uint64_t m, uint64_t d, double inv_d) {
+  const uint64_t q = (uint64_t)((double)m * inv_d);
+  int64_t r = (int64_t)(m - q * d);
+  if (r < 0) r += (int64_t)d;
+  else if (r >= (int64_t)d) r -= (int64_t)d;
+  return (uint64_t)r;
+}
+
 // What a line pass does when it loads a line into LDS.
 enum LoadMode : int32_t {
   LOAD_PLAIN = 0,  // v = x
@@ -167,6 +180,7 @@ struct LinePass {
   // exp(i pi (t^2 s^2 mod 2N) / N) from this FP64 table (L entries; null:
   // read the chirp table instead).
   const double2* chirp_d;
+  double chirp_inv;  // 1 / chirp_n (the register passes' index reductions)
   // Rader passes: LDS slot of each element t < chirp_n of the prime-length
   // line (rtab[0] unused)
   const uint16_t* rtab;

@@ -89,6 +89,14 @@ __device__ __forceinline__ double2 chirp_root(uint64_t m, int64_t N) {
   return make_double2(cs, sn);
 }
 
+// exp(i pi r / N) for r in [0, 2N) (folded into (-N, N] as the chirp table).
+__device__ __forceinline__ double2 chirp_root_r(int64_t r, int64_t N) {
+  if (r > N) r -= 2 * N;
+  double sn, cs;
+  sincospi((double)r / (double)N, &sn, &cs);
+  return make_double2(cs, sn);
+}
+
 // ------------------------------------------------------------------ stages
 
 // Padded position of element i inside an LDS line (one spare slot every
@@ -1175,8 +1183,9 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
   if (chirp_in && p.chirp_d) {
     const int64_t N = p.chirp_n;
     const uint64_t a = (uint64_t)o.cin, st = (uint64_t)p.cae;
-    double2 c = chirp_root(a * a, N);
-    const double2 w = chirp_root(2 * ((a * st) % (uint64_t)N), N);
+    // a, st < N (the planner attaches chirp steps only then): exact reductions
+    double2 c = chirp_root_r((int64_t)mod_q31(a * a, 2 * (uint64_t)N, 0.5 * p.chirp_inv), N);
+    const double2 w = chirp_root_r((int64_t)(2 * mod_q31(a * st, (uint64_t)N, p.chirp_inv)), N);
     // element loads in chunks of kChirpChunk in flight (register budget)
     constexpr int kChirpChunk = 8;
 #pragma unroll
@@ -1243,8 +1252,9 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
   // a chirp store only without one), so this loop holds no twiddle state
   if (store_mode == STORE_CHIRP && p.chirp_d) {
     const uint64_t a = (uint64_t)o.cout, st = (uint64_t)p.cbe;
-    double2 oc = chirp_root(a * a, p.chirp_n);
-    const double2 ow = chirp_root(2 * ((a * st) % (uint64_t)p.chirp_n), p.chirp_n);
+    const uint64_t Nc = (uint64_t)p.chirp_n;
+    double2 oc = chirp_root_r((int64_t)mod_q31(a * a, 2 * Nc, 0.5 * p.chirp_inv), p.chirp_n);
+    const double2 ow = chirp_root_r((int64_t)(2 * mod_q31(a * st, Nc, p.chirp_inv)), p.chirp_n);
 #pragma unroll
     for (int k = 0; k < L; ++k) {
       const bool live = o.cout + (int64_t)k * p.cbe < p.chirp_n;

@@ -1132,6 +1132,22 @@ static fftahip_status attach_chirp_steps(LinePass& p, const Launch& la, Tables& 
   if (cout && p.pretw == 2) return FFTAHIP_SUCCESS;  // the computed store path carries no post-twiddle
   const int64_t step = cin ? p.cae : p.cbe;
   if (step <= 0 || p.chirp_n <= 0) return FFTAHIP_SUCCESS;
+  // the kernel reduces a^2 mod 2N and a*step mod N through an FP64 quotient,
+  // exact when the line start a and the step are below N (quotients < 2^31)
+  {
+    const int64_t* c = cin ? p.ca : p.cb;
+    int64_t amax = 0;
+    int64_t rest = p.nlines;
+    for (int d = 0; d < 3; ++d) {
+      amax += (std::min(p.n[d], rest) - 1) * std::max<int64_t>(0, c[d]);
+      rest = (rest + p.n[d] - 1) / p.n[d];
+      if (c[d] < 0) return FFTAHIP_SUCCESS;
+    }
+    if (c[3] < 0) return FFTAHIP_SUCCESS;
+    amax += (rest - 1) * c[3];
+    if (amax >= p.chirp_n || step >= p.chirp_n) return FFTAHIP_SUCCESS;
+  }
+  p.chirp_inv = 1.0 / (double)p.chirp_n;
   if (g_dry) {
     p.chirp_d = (const double2*)token(TK_CHIRPD, p.L);
     return FFTAHIP_SUCCESS;
@@ -1839,8 +1855,11 @@ struct fftahip_batch_s {
   hipGraphExec_t graph = nullptr;
   const void* graph_in = nullptr;
   void* graph_out = nullptr;
-  // launches run on kStreams streams, ordered only by their dependencies
-  static constexpr int kStreams = 4;
+  // launches run on nstreams (<= kStreams) streams, ordered only by their
+  // dependencies (env FFTAHIP_BATCH_STREAMS, default 4: one per hardware
+  // queue HIP gives a process by default)
+  static constexpr int kStreams = 8;
+  int nstreams = 4;
   hipStream_t streams[kStreams] = {};
   std::vector<hipEvent_t> events;  // one per launch, then fork, then one join per stream
   ~fftahip_batch_s() {
@@ -2713,42 +2732,21 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
       break;
     }
   };
+  // Issue order within a phase: the in-LDS launches first, by kernel variant
+  // (full first) and block 1024 -> 256, then the register launches.  The
+  // one-workgroup-per-CU launches so take CUs as they free up before the
+  // thousands of small register-pass workgroups fill them (measured 3.57 vs
+  // 3.62 ms per step eager, 3.49 vs 3.56 ms as a graph, against register
+  // launches first and blocks 256 -> 1024: env FFTAHIP_BATCH_ORDER=0).
+  const bool old_order = env_i64("FFTAHIP_BATCH_ORDER", 1) == 0;
   for (size_t ph = 0; ph < maxp; ++ph) {
-    // register passes of this phase: one launch per mode.  Within a phase
-    // the full (Bluestein) variants are issued first: they sit on the
-    // longest dependency chains of the launch DAG.
-    for (int mode = 2; mode >= 0; --mode) {
-      fftahip_batch_s::Phase P;
-      P.reg = true;
-      P.block = kRegBlock;
-      P.mode = mode;
-      P.lds = 0;
-      P.ngroups = 0;
-      P.pass_off = b->host_passes.size();
-      P.tile_off = b->host_tiles.size();
-      uint32_t tiles = 0;
-      for (size_t g = 0; g < per_group.size(); ++g) {
-        auto& L = per_group[g];
-        if (ph >= L.size() || !L[ph].reg || L[ph].grid <= 0 || pass_mode(L[ph].lp) != mode) continue;
-        b->host_passes.push_back(L[ph].lp);
-        b->host_tiles.push_back(tiles);
-        tiles += (uint32_t)L[ph].grid;
-        ++P.ngroups;
-        add_deps(P, g, ph);
-        launch_of[g][ph] = (int)b->phases.size();
-      }
-      if (P.ngroups == 0) continue;
-      P.grid = tiles;
-      b->phases.push_back(P);
-    }
-    // in-LDS passes: one launch per (kernel variant, block, LDS class); the
-    // full variants first, as above
-    for (int mode : {kBigVariant + 2, 2, kBigVariant + 1, 1, kBigVariant, 0})
-    for (int block : {256, 512, 1024}) {
-      for (size_t cls : kLdsClass) {
+    auto reg_launches = [&]() {
+      // register passes of this phase: one launch per mode, the full
+      // (Bluestein) variant first
+      for (int mode = 2; mode >= 0; --mode) {
         fftahip_batch_s::Phase P;
-        P.reg = false;
-        P.block = block;
+        P.reg = true;
+        P.block = kRegBlock;
         P.mode = mode;
         P.lds = 0;
         P.ngroups = 0;
@@ -2757,15 +2755,10 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
         uint32_t tiles = 0;
         for (size_t g = 0; g < per_group.size(); ++g) {
           auto& L = per_group[g];
-          if (ph >= L.size() || L[ph].reg || L[ph].block != block || L[ph].grid <= 0 ||
-              lds_class(L[ph].lds) != cls || pass_variant(L[ph].lp) != mode)
-            continue;
-          LinePass lp = L[ph].lp;
-          lp.xcd_remap = 0;
-          b->host_passes.push_back(lp);
+          if (ph >= L.size() || !L[ph].reg || L[ph].grid <= 0 || pass_mode(L[ph].lp) != mode) continue;
+          b->host_passes.push_back(L[ph].lp);
           b->host_tiles.push_back(tiles);
           tiles += (uint32_t)L[ph].grid;
-          P.lds = std::max(P.lds, L[ph].lds);
           ++P.ngroups;
           add_deps(P, g, ph);
           launch_of[g][ph] = (int)b->phases.size();
@@ -2774,11 +2767,56 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
         P.grid = tiles;
         b->phases.push_back(P);
       }
+    };
+    auto lds_launches = [&]() {
+      // in-LDS passes: one launch per (kernel variant, block, LDS class),
+      // the full variants first
+      const std::vector<int> blocks = old_order ? std::vector<int>{256, 512, 1024} : std::vector<int>{1024, 512, 256};
+      for (int mode : {kBigVariant + 2, 2, kBigVariant + 1, 1, kBigVariant, 0})
+        for (int block : blocks)
+          for (size_t cls : kLdsClass) {
+            fftahip_batch_s::Phase P;
+            P.reg = false;
+            P.block = block;
+            P.mode = mode;
+            P.lds = 0;
+            P.ngroups = 0;
+            P.pass_off = b->host_passes.size();
+            P.tile_off = b->host_tiles.size();
+            uint32_t tiles = 0;
+            for (size_t g = 0; g < per_group.size(); ++g) {
+              auto& L = per_group[g];
+              if (ph >= L.size() || L[ph].reg || L[ph].block != block || L[ph].grid <= 0 ||
+                  lds_class(L[ph].lds) != cls || pass_variant(L[ph].lp) != mode)
+                continue;
+              LinePass lp = L[ph].lp;
+              lp.xcd_remap = 0;
+              b->host_passes.push_back(lp);
+              b->host_tiles.push_back(tiles);
+              tiles += (uint32_t)L[ph].grid;
+              P.lds = std::max(P.lds, L[ph].lds);
+              ++P.ngroups;
+              add_deps(P, g, ph);
+              launch_of[g][ph] = (int)b->phases.size();
+            }
+            if (P.ngroups == 0) continue;
+            P.grid = tiles;
+            b->phases.push_back(P);
+          }
+    };
+    if (old_order) {
+      reg_launches();
+      lds_launches();
+    } else {
+      lds_launches();
+      reg_launches();
     }
   }
   if (g_dry) return FFTAHIP_SUCCESS;
   // the streams and events batch_launch orders the launches with
-  const int K = fftahip_batch_s::kStreams;
+  if (!b->streams[0])
+    b->nstreams = (int)std::min<int64_t>(fftahip_batch_s::kStreams, std::max<int64_t>(1, env_i64("FFTAHIP_BATCH_STREAMS", 4)));
+  const int K = b->nstreams;
   if (!b->streams[0])
     for (int k = 0; k < K; ++k) HIP_TRY(hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking));
   if (b->events.size() != b->phases.size() + 1 + K) {
@@ -2805,7 +2843,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
 // Bluestein pass of other lengths) overlap on the device.
 static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_out, hipStream_t s) {
   const BufPtrs bp{d_in, d_out, b->scratch_ptr};
-  const int K = fftahip_batch_s::kStreams;
+  const int K = b->nstreams;
   const size_t np = b->phases.size();
   // streams and events are made by batch_build
   if (!b->streams[0] || b->events.size() != np + 1 + K)
@@ -2815,11 +2853,14 @@ static fftahip_status batch_launch(fftahip_batch b, const void* d_in, void* d_ou
   for (int k = 0; k < K; ++k) HIP_TRY(hipStreamWaitEvent(b->streams[k], fork, 0));
   std::vector<int> stream_of(np, 0);
   int rr = 0;
+  // env FFTAHIP_BATCH_SERIAL=1: every launch on one stream in issue order (A/B:
+  // 4.56 ms per step against 3.57 ms on four streams)
+  static const bool serial = env_i64("FFTAHIP_BATCH_SERIAL", 0) != 0;
   for (size_t i = 0; i < np; ++i) {
     const auto& P = b->phases[i];
     // a launch with a single predecessor follows it on its stream (no event
     // wait); otherwise the streams are taken in turn
-    const int k = P.deps.size() == 1 ? stream_of[P.deps[0]] : (rr++ % K);
+    const int k = serial ? 0 : (P.deps.size() == 1 ? stream_of[P.deps[0]] : (rr++ % K));
     stream_of[i] = k;
     hipStream_t q = b->streams[k];
     for (int d : P.deps)
@@ -3080,7 +3121,15 @@ fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const 
     }
     os << "]";
   }
-  os << "], \"phases\": " << b->phases.size() << "}";
+  os << "], \"phases\": " << b->phases.size() << ", \"launches\": [";
+  for (size_t i = 0; i < b->phases.size(); ++i) {
+    const auto& P = b->phases[i];
+    os << (i ? ", " : "") << "{\"reg\": " << (P.reg ? 1 : 0) << ", \"block\": " << P.block << ", \"mode\": " << P.mode
+       << ", \"lds\": " << P.lds << ", \"grid\": " << P.grid << ", \"groups\": " << P.ngroups << ", \"deps\": [";
+    for (size_t d = 0; d < P.deps.size(); ++d) os << (d ? ", " : "") << P.deps[d];
+    os << "]}";
+  }
+  os << "]}";
   return emit(os.str(), buf, cap, needed);
 }
 
@@ -3360,6 +3409,15 @@ fftahip_status fftahip_debug_chirp_steps(int64_t N, int64_t s, int L, double* ho
   if ((st = d.ensure((size_t)L * sizeof(double2)))) return st;
   HIP_TRY(launch_gen_chirp_steps((double2*)d.p, s, N, L, nullptr));
   HIP_TRY(hipMemcpy(host_out, d.p, (size_t)L * sizeof(double2), hipMemcpyDeviceToHost));
+  return FFTAHIP_SUCCESS;
+}
+
+// Host evaluation of the register passes' 64-bit remainder (mod_q31).
+fftahip_status fftahip_debug_mod_q31(const uint64_t* m, uint64_t d, uint64_t* r, int64_t count) {
+  if (d == 0 || d >= (uint64_t(1) << 32)) return fail(FFTAHIP_ERR_ARGUMENT, "need 0 < d < 2^32");
+  if (count > 0 && (!m || !r)) return fail(FFTAHIP_ERR_ARGUMENT, "NULL argument");
+  const double inv = 1.0 / (double)d;
+  for (int64_t i = 0; i < count; ++i) r[i] = mod_q31(m[i], d, inv);
   return FFTAHIP_SUCCESS;
 }
 

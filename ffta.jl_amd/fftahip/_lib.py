@@ -94,6 +94,8 @@ SIGNATURES = {
     "fftahip_debug_chirp": (_st, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
     "fftahip_debug_chirp_steps": (_st, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double)]),
+    "fftahip_debug_mod_q31": (_st, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]),
     "fftahip_debug_fastdiv": (_st, [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64]),
 }

@@ -283,6 +283,9 @@ fftahip_status fftahip_debug_chirp(int precision, int64_t N, int64_t n0, int64_t
  * ((t s mod 2N)^2 mod 2N) / N) for t < L (interleaved re, im).  Needs a
  * device. */
 fftahip_status fftahip_debug_chirp_steps(int64_t N, int64_t s, int L, double* host_out);
+/* Host evaluation of the register passes' 64-bit remainder m mod d (d <
+ * 2^32, m < 2^62 with m / d < 2^31) through an FP64 quotient. */
+fftahip_status fftahip_debug_mod_q31(const uint64_t* m, uint64_t d, uint64_t* r, int64_t count);
 /* Host evaluation of the kernels' division by a runtime constant d. */
 fftahip_status fftahip_debug_fastdiv(uint32_t d, const uint32_t* n, uint32_t* q, int64_t count);
 

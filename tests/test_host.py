@@ -386,6 +386,40 @@ def test_autotune_tile_table(tmp_path):
         autotune.clear()
 
 
+def test_mod_q31_exact():
+    """The register passes reduce a^2 mod 2N and a*s mod N (a, s < N < 2^31)
+    through an FP64 quotient (mod_q31); it must equal the exact remainder,
+    including next to multiples of d where the quotient is off by one."""
+    import ctypes
+
+    lib = _lib.load()
+    rng = np.random.default_rng(7)
+    cases = []
+    for N in [1, 2, 3, 73, 1009, 65521, 1000003, (1 << 26) - 5, (1 << 30) + 3, (1 << 31) - 19, (1 << 31) - 1]:
+        a = np.concatenate([rng.integers(0, N, 200), [0, 1, N - 1, max(0, N - 2), N // 2]]).astype(object)
+        s = np.concatenate([rng.integers(0, N, 200), [N - 1, 1, 0, N // 3, max(1, N - 7)]]).astype(object)
+        cases.append((2 * N, [int(v) * int(v) for v in a]))
+        cases.append((N, [int(x) * int(y) for x, y in zip(a, s)]))
+        # multiples of d and their neighbours, quotients up to 2^31 - 1
+        d = 2 * N
+        k = np.concatenate([rng.integers(1, max(2, min(N, 1 << 31)), 100), [1, max(1, min(N, (1 << 31) - 1) - 1)]])
+        near = []
+        for q in k:
+            for e in (-1, 0, 1):
+                m = int(q) * d + e
+                if 0 <= m < (1 << 62) and m // d < (1 << 31):
+                    near.append(m)
+        cases.append((d, near))
+    for d, ms in cases:
+        m = np.array(ms, dtype=np.uint64)
+        r = np.zeros_like(m)
+        st = lib.fftahip_debug_mod_q31(m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), d,
+                                       r.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), m.size)
+        assert st == 0
+        exact = np.array([int(v) % d for v in ms], dtype=np.uint64)
+        assert np.array_equal(r, exact), (d, [(int(a), int(b), int(c)) for a, b, c in zip(m, r, exact) if b != c][:5])
+
+
 def test_debug_chirp_argument_checks():
     """The chirp read-back entry points refuse bad arguments before touching a device."""
     import ctypes
