@@ -2740,10 +2740,10 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
   // launches first and blocks 256 -> 1024: env FFTAHIP_BATCH_ORDER=0).
   const bool old_order = env_i64("FFTAHIP_BATCH_ORDER", 1) == 0;
   for (size_t ph = 0; ph < maxp; ++ph) {
-    auto reg_launches = [&]() {
+    auto reg_launches = [&](int mode_hi, int mode_lo) {
       // register passes of this phase: one launch per mode, the full
       // (Bluestein) variant first
-      for (int mode = 2; mode >= 0; --mode) {
+      for (int mode = mode_hi; mode >= mode_lo; --mode) {
         fftahip_batch_s::Phase P;
         P.reg = true;
         P.block = kRegBlock;
@@ -2805,11 +2805,11 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
           }
     };
     if (old_order) {
-      reg_launches();
+      reg_launches(2, 0);
       lds_launches();
     } else {
       lds_launches();
-      reg_launches();
+      reg_launches(2, 0);
     }
   }
   if (g_dry) return FFTAHIP_SUCCESS;
