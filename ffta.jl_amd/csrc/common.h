@@ -16,6 +16,11 @@ __host__ __device__ constexpr bool prime_codelet(int r, int prec) {
   return prec == 32 && (r == 17 || r == 19 || r == 23);
 }
 __host__ __device__ constexpr bool big_stage(int r, int prec) { return r >= 17 && !prime_codelet(r, prec); }
+// Codelet radices of 5-smooth convolution lines whose first / last stage has
+// a zero-padded variant (inputs or outputs t >= (r + 1) / 2 dropped).
+__host__ __device__ constexpr bool zero_pad_radix(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 9 || r == 16;
+}
 constexpr int kMaxLines = 256;      // lines per workgroup, at most
 constexpr int kRegBlock = 256;      // lines (threads) per workgroup of a register line pass
 // per-line offset tables of a line-pass tile, kept in LDS after its lines:
@@ -135,6 +140,14 @@ struct LinePass {
   // 1: every line of a tile lies within one wavefront during the sweeps
   // (threads per line divide 64), so their stages synchronise per wave
   int32_t wave_sync;
+  // zero-padded convolution lines (one-kernel Bluestein): bit 0, the line is
+  // zero at positions >= zero_from, where zero_from = ((r + 1) / 2) * (L / r)
+  // for the first radix r, and the first sweep's first stage skips those
+  // inputs (the load phase writes positions < zero_from only); bit 1, the
+  // store reads positions < ((r + 1) / 2) * (L / r) only, r the last radix,
+  // and the second sweep's last stage computes and writes no others
+  int32_t zero_pad;
+  int32_t zero_from;
   FastDiv divC, divL;       // by C and by L
   FastDiv divR;             // by block / C
   int64_t nlines;           // < 2^31
@@ -173,6 +186,10 @@ struct LinePass {
   int64_t chirp_n;
   int64_t ca[4], cb[4], cm[4];
   int64_t cae, cbe, cme;  // spectrum index in the middle of a pass: n = sum_d i_d*cm[d] + k*cme
+  // > 0: lines of dimension 0 are the rows k1 < mirror_rows of a row-permuted
+  // spectrum, of which only rows k1 <= mirror_rows / 2 are stored; a row
+  // k1 above that reads row mirror_rows - k1 backwards (B[M - k] = B[k])
+  int32_t mirror_rows;
   const void* chirp;  // chirp_n entries, pass precision
   const void* bhat;   // spectrum factor (mid: L entries; store: indexed by n)
   // Register passes: the chirp of a line's positions n = a + t*s is formed as

@@ -129,10 +129,19 @@ __device__ __forceinline__ void line_sync(bool wave) {
 // padded (i + (i >> sh)); when a stride is a multiple of 2^sh the padded
 // positions of a butterfly are an arithmetic sequence and are stepped
 // instead of recomputed.
-template <class Rt, int RAD, int EPT, bool FUSE, class LdF, class StF>
+// Zero-padded convolution lines (ZM, compile time): ZM == ZM_IN is the first
+// stage of a sweep over a line that is zero at positions >= KH * L / RAD,
+// KH = (RAD + 1) / 2: its inputs t >= KH are taken as zero without an LDS
+// read (and the butterfly folds them away); ZM == ZM_OUT is the last stage
+// of a sweep whose outputs are read at positions < KH * L / RAD only: the
+// outputs t >= KH are not computed or written.
+enum ZeroMode { ZM_NONE = 0, ZM_IN = 1, ZM_OUT = 2 };
+
+template <class Rt, int RAD, int EPT, bool FUSE, int ZM = ZM_NONE, class LdF, class StF>
 __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl, int L, int Ns, FastDiv nsd,
                                       const cx<Rt>* __restrict__ tw, int sh, bool gload, bool gstore, const LdF& ld,
                                       const StF& st, const cx<Rt>* __restrict__ pm, int pms, bool ws) {
+  constexpr int KH = (RAD + 1) / 2;
   // butterflies per thread (a prime codelet above EPT points: one)
   constexpr int NB = EPT / RAD > 0 ? EPT / RAD : 1;
   const int Lr = L / RAD;
@@ -192,6 +201,11 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       if (FUSE && gload) {
 #pragma unroll
         for (int t = 0; t < RAD; ++t) v[b][t] = ld(j + t * Lr);
+      } else if (ZM == ZM_IN) {
+        const int p0 = pad_idx(j, she);
+#pragma unroll
+        for (int t = 0; t < RAD; ++t)
+          v[b][t] = t < KH ? (rstep ? line[p0 + t * rstride] : line[pad_idx(j + t * Lr, she)]) : cx<Rt>{(Rt)0, (Rt)0};
       } else if (rstep) {
         const int p0 = pad_idx(j, she);
 #pragma unroll
@@ -221,10 +235,12 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
       } else if (wstep) {
         const int p0 = pad_idx(base, she);
 #pragma unroll
-        for (int t = 0; t < RAD; ++t) line[p0 + t * wstride] = v[b][t];
+        for (int t = 0; t < RAD; ++t)
+          if (ZM != ZM_OUT || t < KH) line[p0 + t * wstride] = v[b][t];
       } else {
 #pragma unroll
-        for (int t = 0; t < RAD; ++t) line[pad_idx(base + t * Ns, she)] = v[b][t];
+        for (int t = 0; t < RAD; ++t)
+          if (ZM != ZM_OUT || t < KH) line[pad_idx(base + t * Ns, she)] = v[b][t];
       }
     }
   }
@@ -408,13 +424,15 @@ __device__ __forceinline__ void stage_mid(cx<Rt>* line, int jl, bool active, int
 // All radix stages of a line.  With fuse_first the first stage reads its
 // inputs through ld(element) from global memory instead of LDS; with
 // fuse_last the last stage writes through st(element, value).  Stages
-// [s_begin, s_end) run (default: all).
-template <class Rt, int EPT, bool FUSE, bool BIG, class LdF, class StF>
+// [s_begin, s_end) run (default: all).  ZP (compile time) carries the
+// zero-padded first / last stage variants, taken at run time by zero_first
+// (stage 0) and prune_last (stage nst - 1).
+template <class Rt, int EPT, bool FUSE, bool BIG, bool ZP = false, class LdF, class StF>
 __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const LinePass& p,
                                       const cx<Rt>* __restrict__ tw, const cx<Rt>* roots, bool fuse_first,
                                       bool fuse_last, const LdF& ld, const StF& st,
                                       const cx<Rt>* __restrict__ pm = nullptr, int pms = 0, int s_begin = 0,
-                                      int s_end = -1) {
+                                      int s_end = -1, bool zero_first = false, bool prune_last = false) {
   int Ns = 1;
   const int sh = p.lds_sh;
   const bool ws = p.wave_sync != 0;
@@ -440,7 +458,19 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
     }
     switch (r) {
 #define FFTAHIP_STAGE(R) \
-  case R: stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == s_begin ? pm : nullptr, pms, ws); break;
+  case R: \
+    if constexpr (ZP && zero_pad_radix(R)) { \
+      if (zero_first && s == 0) { \
+        stage<Rt, R, EPT, FUSE, ZM_IN>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, nullptr, 0, ws); \
+        break; \
+      } \
+      if (prune_last && s == p.nst - 1) { \
+        stage<Rt, R, EPT, FUSE, ZM_OUT>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == s_begin ? pm : nullptr, pms, ws); \
+        break; \
+      } \
+    } \
+    stage<Rt, R, EPT, FUSE>(line, jl, active, p.Tl, p.L, Ns, nsd, tw, sh, gl, gs, ld, st, s == s_begin ? pm : nullptr, pms, ws); \
+    break;
       FFTAHIP_STAGE(2)
       FFTAHIP_STAGE(3)
       FFTAHIP_STAGE(4)
@@ -488,6 +518,14 @@ struct LineShared {
   int64_t *in, *out, *twk, *cin, *cout, *cmid;
 };
 
+// Spectrum position of a line's first element and its step along the line,
+// from the tile's table entry (complemented for a mirrored row).
+__device__ __forceinline__ void mid_ref(int64_t v, int64_t cme, int64_t& base, int64_t& step) {
+  const bool mir = v < 0;
+  base = mir ? ~v : v;
+  step = mir ? -cme : cme;
+}
+
 // Kernel variants (compile-time), so that the common passes carry no
 // register cost for operations they do not use:
 //   MODE_PLAIN  load, radix sweeps, store (optionally conjugating / scaling);
@@ -510,6 +548,7 @@ __device__ __forceinline__ double2 dmul2(double2 a, double2 b) {
 // line index is split over n[0], n[1], n[2] and a last (table) dimension.
 struct LineOffsets {
   int64_t in, out, twk, cin, cout, cmid;
+  int64_t cms;  // direction of the spectrum row (+1, or -1 for a mirrored row)
 };
 
 template <int MODE>
@@ -528,6 +567,7 @@ __device__ __forceinline__ LineOffsets line_offsets(const LinePass& p, uint32_t 
   if (p.in_tab) o.in += gld(p.in_tab + idx[3]);
   if (p.out_tab) o.out += gld(p.out_tab + idx[3]);
   o.twk = o.cin = o.cout = o.cmid = 0;
+  o.cms = 1;
   if (MODE != MODE_PLAIN) {
     const int d = p.twdim & 3;
     o.twk = d == 0 ? idx[0] : (d == 1 ? idx[1] : (d == 2 ? idx[2] : idx[3]));
@@ -536,6 +576,13 @@ __device__ __forceinline__ LineOffsets line_offsets(const LinePass& p, uint32_t 
     o.cin = idx[0] * p.ca[0] + idx[1] * p.ca[1] + idx[2] * p.ca[2] + idx[3] * p.ca[3];
     o.cout = idx[0] * p.cb[0] + idx[1] * p.cb[1] + idx[2] * p.cb[2] + idx[3] * p.cb[3];
     o.cmid = idx[0] * p.cm[0] + idx[1] * p.cm[1] + idx[2] * p.cm[2] + idx[3] * p.cm[3];
+    // mirrored spectrum rows (the three-pass row pass): row k1 > M1/2 reads
+    // row M1 - k1 backwards, B[k1 + M1 k2] = B[M - k1 - M1 k2] = row (M1 - k1)
+    // at M2 - 1 - k2 (the Bluestein spectrum is symmetric, B[M - k] = B[k])
+    if (p.mirror_rows && 2 * idx[0] > p.mirror_rows) {
+      o.cmid += (p.mirror_rows - 2 * idx[0]) * p.cm[0] + (int64_t)(p.L - 1) * p.cme;
+      o.cms = -1;
+    }
   }
   return o;
 }
@@ -621,7 +668,8 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     if (MODE == MODE_FULL) {
       sh.cin[c] = o.cin;
       sh.cout[c] = o.cout;
-      sh.cmid[c] = o.cmid;
+      // a mirrored row keeps its offset complemented (the offsets are >= 0)
+      sh.cmid[c] = o.cms < 0 ? ~o.cmid : o.cmid;
     }
   }
   __syncthreads();
@@ -647,6 +695,12 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const bool fuse_in = MODE == MODE_PLAIN && p.fuse_in != 0;
   const bool fuse_out = MODE == MODE_PLAIN && p.fuse_out != 0;
   const bool has_mid = MODE == MODE_FULL && p.mid_mode == MID_BHAT;
+  // zero-padded Bluestein lines (5-smooth, so never the direct-DFT variant):
+  // the line is zero from p.zero_from on, and the output is read below the
+  // last stage's pruning bound
+  constexpr bool kZero = MODE == MODE_FULL && !BIG;
+  const bool zero_in = kZero && (p.zero_pad & 1) != 0;
+  const bool prune_out = kZero && (p.zero_pad & 2) != 0;
 
   // FP32 passes take the pass twiddles along a thread's run of positions as
   // powers of one FP64 ratio (see reg_line); FP64 passes look each one up.
@@ -719,7 +773,10 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         w = step_twiddle(p, sh.twk[c] * (int64_t)in_t0);
         if (tstep < L) wr = step_twiddle(p, sh.twk[c] * (int64_t)tstep);
       }
-      for (int t0 = in_t0; t0 < L; t0 += tstep * U) {
+      // a zero-padded line is written up to zero_from (the first stage reads
+      // no position past it)
+      const int lend = zero_in ? p.zero_from : L;
+      for (int t0 = in_t0; t0 < lend; t0 += tstep * U) {
         cx<Rt> v[U], f[U];
         PassTw<Rt> pw[U];
 #pragma unroll
@@ -738,7 +795,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int t = t0 + u * tstep;
-          if (t >= L) continue;
+          if (t >= lend) continue;
           cx<Rt> x = cconj_if(v[u], sgn_in);
           if (chirp_in) x = cmul(x, cconj(f[u]));
           if (pretw_in) {
@@ -847,8 +904,8 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   // differently (the separate spectrum phase, the store) need the workgroup
   // barrier the sweeps' last stage no longer gives
   const bool ws = p.wave_sync != 0;
-  sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, fuse_in, fuse_out && !has_mid, ld, st, nullptr, 0, 0,
-                             mid_fused ? p.nst - 1 : p.nst);
+  sweep<Rt, EPT, kFuse, BIG, kZero>(myline, jl, active, p, tw, roots, fuse_in, fuse_out && !has_mid, ld, st, nullptr,
+                                    0, 0, mid_fused ? p.nst - 1 : p.nst, zero_in, false);
   if (rader && !mid_fused) {
     // the DC term of the first sweep (sum of elements 1..P-1) before the
     // spectrum product overwrites it
@@ -861,10 +918,12 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
     // stage in one LDS round trip (the planner sets mid_fused only when the
     // schedule's first and last radix are the same codelet radix)
     const int rdc = rader ? ldl - kRaderSide + 1 : -1;
-    const cx<Rt>* __restrict__ pm = bhat + sh.cmid[cs];
+    int64_t mb, ms;
+    mid_ref(sh.cmid[cs], p.cme, mb, ms);
+    const cx<Rt>* __restrict__ pm = bhat + mb;
     switch (p.rad[p.nst - 1]) {
 #define FFTAHIP_MIDSTAGE(R) \
-  case R: stage_mid<Rt, R, EPT>(myline, jl, active, p.Tl, p.L, tw, p.lds_sh, pm, (int)p.cme, rdc, ws); break;
+  case R: stage_mid<Rt, R, EPT>(myline, jl, active, p.Tl, p.L, tw, p.lds_sh, pm, (int)ms, rdc, ws); break;
       FFTAHIP_MIDSTAGE(2)
       FFTAHIP_MIDSTAGE(3)
       FFTAHIP_MIDSTAGE(4)
@@ -878,16 +937,22 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
 #undef FFTAHIP_MIDSTAGE
       default: break;
     }
-    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, nullptr, 0, 1);
+    sweep<Rt, EPT, kFuse, BIG, kZero>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, nullptr, 0, 1, -1, false,
+                                      prune_out);
   } else if (has_mid && p.mid_in_stage) {
-    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, bhat + sh.cmid[cs], (int)p.cme);
+    int64_t mb, ms;
+    mid_ref(sh.cmid[cs], p.cme, mb, ms);
+    sweep<Rt, EPT, kFuse, BIG, kZero>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, bhat + mb, (int)ms, 0, -1,
+                                      false, prune_out);
   } else if (has_mid) {
     if (ws) __syncthreads();
     if (p.row_mid && col_step) {
       if (row_c < nvalid) {
         const int c = row_c;
-        const cx<Rt>* __restrict__ fp = bhat + sh.cmid[c] + (int64_t)row_t0 * p.cme;
-        const int64_t fstep = (int64_t)tstep * p.cme;
+        int64_t mb, ms;
+        mid_ref(sh.cmid[c], p.cme, mb, ms);
+        const cx<Rt>* __restrict__ fp = bhat + mb + (int64_t)row_t0 * ms;
+        const int64_t fstep = (int64_t)tstep * ms;
         cx<Rt>* line = lds + c * ldl;
         for (int t0 = row_t0; t0 < L; t0 += tstep * U) {
           cx<Rt> f[U], x[U];
@@ -920,7 +985,9 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
           const int t = e - c * L;
           if (c < nvalid) {
             slot[u] = c * ldl + pad_idx(t, pshe);
-            f[u] = gld(bhat + sh.cmid[c] + (int64_t)t * p.cme);
+            int64_t mb, ms;
+            mid_ref(sh.cmid[c], p.cme, mb, ms);
+            f[u] = gld(bhat + mb + (int64_t)t * ms);
           }
         }
       }
@@ -929,7 +996,8 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         if (slot[u] >= 0) lds[slot[u]] = cconj(cmul(lds[slot[u]], f[u]));
     }
     __syncthreads();
-    sweep<Rt, EPT, kFuse, BIG>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st);
+    sweep<Rt, EPT, kFuse, BIG, kZero>(myline, jl, active, p, tw, roots, false, fuse_out, ld, st, nullptr, 0, 0, -1, false,
+                                      prune_out);
   }
 
   if (ws) __syncthreads();
@@ -1025,7 +1093,7 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
         w = step_twiddle(p, sh.twk[c] * (int64_t)ot0);
         if (tstep < L) wr = step_twiddle(p, sh.twk[c] * (int64_t)tstep);
       }
-      for (int t0 = ot0; t0 < L; t0 += tstep * U) {
+      for (int t0 = ot0; t0 < tlive; t0 += tstep * U) {
         cx<Rt> x[U], f[U];
         PassTw<Rt> pw[U];
 #pragma unroll

@@ -75,7 +75,8 @@ hipError_t launch_gen_chirp_range(void* out, int precision, int64_t n0, int64_t 
 hipError_t launch_gen_chirp_ext(double2* bext, const double2* chirp64, int64_t N, int64_t M, hipStream_t stream);
 // Rader's convolution kernel b_r = exp(-2 pi i g^r / P), r < P - 1, FP64.
 hipError_t launch_gen_rader_b(double2* b, int64_t P, int64_t g, hipStream_t stream);
-hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, hipStream_t stream);
+hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, int64_t rows,
+                               hipStream_t stream);
 hipError_t launch_finish_bhat(void* bhat, const double2* spec64, int precision, int64_t M, hipStream_t stream);
 
 }  // namespace fftahip

@@ -299,9 +299,9 @@ __global__ void finish_bhat_kernel(cx<Rt>* bhat, const double2* spec64, int64_t 
 // dst[k1*M2 + k2] = src[k1 + M1*k2]: the spectrum in the order the fused
 // Bluestein row pass reads it (row k1 contiguous).
 template <class Rt>
-__global__ void permute_rows_kernel(cx<Rt>* dst, const cx<Rt>* src, int64_t M1, int64_t M2) {
+__global__ void permute_rows_kernel(cx<Rt>* dst, const cx<Rt>* src, int64_t M1, int64_t M2, int64_t rows) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M1 * M2) return;
+  if (i >= rows * M2) return;
   const int64_t k1 = i / M2, k2 = i - k1 * M2;
   dst[i] = src[k1 + M1 * k2];
 }
@@ -500,14 +500,15 @@ hipError_t launch_gen_chirp_ext(double2* bext, const double2* chirp64, int64_t N
   return hipGetLastError();
 }
 
-hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, hipStream_t stream) {
-  const int64_t blocks = (M1 * M2 + 255) / 256;
+hipError_t launch_permute_rows(void* dst, const void* src, int precision, int64_t M1, int64_t M2, int64_t rows,
+                               hipStream_t stream) {
+  const int64_t blocks = (rows * M2 + 255) / 256;
   if (precision == 32)
     hipLaunchKernelGGL(permute_rows_kernel<float>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                       reinterpret_cast<cx<float>*>(dst), reinterpret_cast<const cx<float>*>(src), M1, M2);
+                       reinterpret_cast<cx<float>*>(dst), reinterpret_cast<const cx<float>*>(src), M1, M2, rows);
   else
     hipLaunchKernelGGL(permute_rows_kernel<double>, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                       reinterpret_cast<cx<double>*>(dst), reinterpret_cast<const cx<double>*>(src), M1, M2);
+                       reinterpret_cast<cx<double>*>(dst), reinterpret_cast<const cx<double>*>(src), M1, M2, rows);
   return hipGetLastError();
 }
 

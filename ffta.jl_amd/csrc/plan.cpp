@@ -1193,6 +1193,32 @@ static void set_out(LinePass& p, BufRef r) {
 // Operations of a Rader line pass over the prime lp.bq: load through the
 // slot table, spectrum product between the sweeps, x0 + conj on store; the
 // spectrum is indexed by the position alone (one table for every line).
+// A Bluestein line of L >= 2 chirp_n - 1 points holds its chirp_n inputs at
+// positions t < chirp_n and zeros after them, and its store reads positions
+// t < chirp_n only.  When chirp_n <= ((r + 1) / 2) * (L / r) for the first
+// radix r, the first stage drops its inputs t >= (r + 1) / 2 (all zero) and
+// the load writes no position past that bound; when the same holds for the
+// last radix, the last stage drops its outputs t >= (r + 1) / 2.  Needs
+// every line to start at chirp index 0 with unit steps on both sides.
+static void set_zero_pad(LinePass& p, const Launch& la, int prec) {
+  p.zero_pad = 0;
+  p.zero_from = 0;
+  // a one-stage fused boundary (mid_fused, nst == 1) runs no sweep stage
+  if (la.reg || p.nst < 1 || (p.mid_fused && p.nst < 2) || env_i64("FFTAHIP_ZERO_PAD", 1) == 0) return;
+  if (p.load_mode != LOAD_CHIRP || p.mid_mode != MID_BHAT || p.store_mode != STORE_CHIRP || p.pretw != 0) return;
+  if (p.cae != 1 || p.cbe != 1 || p.mirror_rows) return;
+  for (int u = 0; u < 4; ++u)
+    if (p.ca[u] != 0 || p.cb[u] != 0) return;
+  const int r0 = p.rad[0], rl = p.rad[p.nst - 1];
+  const int64_t f0 = (int64_t)((r0 + 1) / 2) * (p.L / r0);
+  const int64_t fl = (int64_t)((rl + 1) / 2) * (p.L / rl);
+  if (codelet_radix(r0, prec) && zero_pad_radix(r0) && p.chirp_n <= f0) {
+    p.zero_pad |= 1;
+    p.zero_from = (int32_t)f0;
+  }
+  if (codelet_radix(rl, prec) && zero_pad_radix(rl) && p.chirp_n <= fl) p.zero_pad |= 2;
+}
+
 static void set_rader(LinePass& p, const LengthPlan& lp) {
   p.load_mode = LOAD_RADER;
   p.mid_mode = MID_BHAT;
@@ -1252,6 +1278,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       set_rader(p, lp);
     }
     set_inner(p, d->inner[0], prec, la, false);
+    set_zero_pad(p, la, prec);
     la.lp = p;
     L.push_back(la);
     return FFTAHIP_SUCCESS;
@@ -1316,6 +1343,8 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
         p.mid_mode = MID_BHAT;
         p.cme = 1;
         p.bhat = lp.bhat_rows_ptr;
+        // rows above M1/2 read their mirror row backwards (half the table)
+        p.mirror_rows = (int32_t)M1;
       } else {
         set_in(p, s0);
         set_out(p, out);
@@ -1431,6 +1460,7 @@ static fftahip_status build_launches(LengthPlan& lp, Tables& T, const LineSet& l
       for (int u = 0; u < 4; ++u) p.ca[u] = p.cb[u] = p.cm[u] = 0;
     }
     set_inner(p, d->inner[s], prec, la);
+    set_zero_pad(p, la, prec);
     if ((st = attach_pass_twiddle(p, la, T, M, stream))) return st;
     if ((st = attach_chirp_steps(p, la, T, stream))) return st;
     la.lp = p;
@@ -1607,8 +1637,11 @@ static fftahip_status prepare_bluestein(LengthPlan& lp, Tables& T, hipStream_t s
   HIP_TRY(launch_finish_bhat(lp.bhat.p, (const double2*)spec.p, lp.prec, M, s));
   int64_t M1, M2;
   if (!lp.mixed && blue3_split(M, lp.prec, &M1, &M2)) {
-    if ((st = lp.bhat_rows.ensure((size_t)M * csize(lp.prec)))) return st;
-    HIP_TRY(launch_permute_rows(lp.bhat_rows.p, lp.bhat.p, lp.prec, M1, M2, s));
+    // rows 0 .. M1/2 of the row-permuted spectrum; the row pass reads the
+    // others mirrored (B[M - k] = B[k])
+    const int64_t rows = M1 / 2 + 1;
+    if ((st = lp.bhat_rows.ensure((size_t)(rows * M2) * csize(lp.prec)))) return st;
+    HIP_TRY(launch_permute_rows(lp.bhat_rows.p, lp.bhat.p, lp.prec, M1, M2, rows, s));
     lp.bhat_rows_ptr = lp.bhat_rows.p;
   }
   HIP_TRY(hipStreamSynchronize(s));
@@ -3048,7 +3081,7 @@ static void json_launch(std::ostringstream& os, const Launch& la, int prec) {
   os << ", ";
   json_ptr(os, "tw_hi", p.tw_hi);
   os << ", \"conj_in\": " << p.conj_in << ", \"conj_out\": " << p.conj_out << ", \"scale\": " << p.scale
-     << ", \"load_mode\": " << p.load_mode << ", \"mid_mode\": " << p.mid_mode << ", \"mid_in_stage\": " << p.mid_in_stage << ", \"mid_fused\": " << p.mid_fused << ", \"wave_sync\": " << p.wave_sync << ", \"store_mode\": " << p.store_mode
+     << ", \"load_mode\": " << p.load_mode << ", \"mid_mode\": " << p.mid_mode << ", \"mirror_rows\": " << p.mirror_rows << ", \"mid_in_stage\": " << p.mid_in_stage << ", \"mid_fused\": " << p.mid_fused << ", \"wave_sync\": " << p.wave_sync << ", \"zero_pad\": " << p.zero_pad << ", \"zero_from\": " << p.zero_from << ", \"store_mode\": " << p.store_mode
      << ", \"xcd_remap\": " << p.xcd_remap << ", \"chirp_n\": " << p.chirp_n << ", \"ca\": [" << p.ca[0] << ", "
      << p.ca[1] << ", " << p.ca[2] << ", " << p.ca[3] << "], \"cb\": [" << p.cb[0] << ", " << p.cb[1] << ", "
      << p.cb[2] << ", " << p.cb[3] << "], \"cm\": [" << p.cm[0] << ", " << p.cm[1] << ", " << p.cm[2] << ", "

@@ -22,6 +22,9 @@ EPT = 16
 # FP32 direct-DFT kernel variants, one butterfly per thread
 CODELETS = {2, 3, 4, 5, 7, 8, 9, 11, 13, 16}
 PRIME_CODELETS = {17, 19, 23}
+# codelet radices with zero-padded first / last stage variants (common.h
+# zero_pad_radix): inputs / outputs t >= (r + 1) // 2 dropped
+ZERO_PAD_RADICES = {2, 3, 4, 5, 8, 9, 16}
 
 
 def codelets(esize: int) -> set:
@@ -140,20 +143,38 @@ def _pad(i, sh):
     return i + (i >> sh) if sh else i
 
 
-def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None, cod=CODELETS) -> None:
+def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None, cod=CODELETS, zero_first: bool = False,
+                  prune_last: bool = False) -> None:
     """Replay the kernel's stage loop for one line with its exact LDS
     positions (padding, stepped addressing) and table indexing; raise if an
-    output slot is missed or written twice, or if the result is not the DFT."""
+    output slot is missed or written twice, or if the result is not the DFT.
+
+    zero_first: the line is zero from ((r + 1) // 2) * (L // r) on (r the
+    first radix), those LDS slots hold garbage (NaN) and the first stage takes
+    its inputs t >= (r + 1) // 2 as zero without reading them.  prune_last:
+    the last stage writes its outputs t < (r + 1) // 2 only, and only the
+    positions below ((r + 1) // 2) * (L // r) are checked."""
     rng = rng or np.random.default_rng(0)
     x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
     tw = stage_table(L, rad, cod)
     P = _pad(L - 1, sh) + 1
     lds = np.full(P, np.nan + 0j)
-    lds[_pad(np.arange(L), sh)] = x
+    zf = L
+    if zero_first:
+        if rad[0] not in ZERO_PAD_RADICES or rad[0] not in cod:
+            raise ScheduleError(f"zero-padded first stage of radix {rad[0]}")
+        zf = (rad[0] + 1) // 2 * (L // rad[0])
+        x[zf:] = 0
+    lds[_pad(np.arange(zf), sh)] = x[:zf]
+    keep = L
+    if prune_last:
+        if rad[-1] not in ZERO_PAD_RADICES or rad[-1] not in cod:
+            raise ScheduleError(f"pruned last stage of radix {rad[-1]}")
+        keep = (rad[-1] + 1) // 2 * (L // rad[-1])
     pmask = (1 << sh) - 1 if sh else 0
     Ns = 1
     roots_off = L - 1
-    for R in rad:
+    for s_i, R in enumerate(rad):
         Lr = L // R
         if R not in cod:
             new, hits = _big_stage(lds, tw, L, R, Ns, Tl, sh, roots_off, P)
@@ -184,7 +205,9 @@ def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None, cod=CODELETS) -> 
                 src = _pad(j, sh)[:, None] + t[None, :] * rstride
             else:
                 src = _pad(j[:, None] + t[None, :] * Lr, sh)
+            kin = (R + 1) // 2 if (zero_first and s_i == 0) else R
             v = lds[src]
+            v[:, kin:] = 0
             k = j % Ns
             if Ns > 1:
                 widx = (Ns - 1) + k[:, None] * (R - 1) + t[None, 1:] - 1
@@ -199,18 +222,20 @@ def emulate_sweep(L: int, rad, Tl: int, sh: int = 0, rng=None, cod=CODELETS) -> 
                 dst = _pad(base[:, None] + t[None, :] * Ns, sh)
             if dst.max() >= P:
                 raise ScheduleError("stage write out of range")
-            new[dst] = V
-            np.add.at(hits, dst.ravel(), 1)
-        if not np.all(hits[_pad(np.arange(L), sh)] == 1) or hits.sum() != L:
+            kout = (R + 1) // 2 if (prune_last and s_i == len(rad) - 1) else R
+            new[dst[:, :kout]] = V[:, :kout]
+            np.add.at(hits, dst[:, :kout].ravel(), 1)
+        nw = keep if (prune_last and s_i == len(rad) - 1) else L
+        if not np.all(hits[_pad(np.arange(nw), sh)] == 1) or hits.sum() != nw:
             raise ScheduleError(f"L={L} stage radix {R}: slots written {np.unique(hits)} times")
         lds = new
         Ns *= R
     if Ns != L:
         raise ScheduleError(f"radices {rad} multiply to {Ns}, not {L}")
-    line = lds[_pad(np.arange(L), sh)]
-    ref = np.fft.fft(x)
+    line = lds[_pad(np.arange(keep), sh)]
+    ref = np.fft.fft(x)[:keep]
     err = np.linalg.norm(line - ref) / max(np.linalg.norm(ref), 1e-300)
-    if err > 1e-12:
+    if not err <= 1e-12:  # NaN: a slot read before it was written
         raise ScheduleError(f"sweep of L={L} radices {rad} wrong (rel err {err})")
 
 
@@ -254,6 +279,26 @@ class Emulator:
         if amax * 16 > tok[1]:
             raise ScheduleError(f"pass twiddle second factor: needs {amax * 16} entries, table has {tok[1]}")
 
+    @staticmethod
+    def _check_zero_pad(p, zp):
+        """A zero-padded line (plan.cpp set_zero_pad) must be a Bluestein line
+        whose inputs and read outputs all lie below the stage bounds."""
+        L, rad = p["L"], p["rad"]
+        if not (p["load_mode"] == 1 and p["mid_mode"] == 1 and p["store_mode"] == 2 and p["cae"] == 1
+                and p["cbe"] == 1 and not p["pretw"] and not any(p["ca"]) and not any(p["cb"])
+                and not p.get("mirror_rows", 0)):
+            raise ScheduleError("zero-padded pass that is not a chirp-loaded Bluestein line")
+        if p.get("mid_fused") and len(rad) < 2:
+            raise ScheduleError("zero-padded pass with a one-stage fused boundary")
+        if zp & 1:
+            f0 = (rad[0] + 1) // 2 * (L // rad[0])
+            if rad[0] not in ZERO_PAD_RADICES or p.get("zero_from") != f0 or p["chirp_n"] > f0:
+                raise ScheduleError(f"zero tail from {p.get('zero_from')} (radix {rad[0]}, {p['chirp_n']} inputs)")
+        if zp & 2:
+            fl = (rad[-1] + 1) // 2 * (L // rad[-1])
+            if rad[-1] not in ZERO_PAD_RADICES or p["chirp_n"] > fl:
+                raise ScheduleError(f"pruned last stage of radix {rad[-1]} below {p['chirp_n']} outputs")
+
     def _pass(self, p, lds_per_block):
         L, C, Tl = p["L"], p["C"], p["Tl"]
         if p.get("reg"):
@@ -288,9 +333,18 @@ class Emulator:
                 raise ScheduleError("grid does not cover all lines")
             if p["xcd_remap"] and p["grid"] % 8:
                 raise ScheduleError("XCD remap needs grid % 8 == 0")
-            key = (L, tuple(p["rad"]), Tl, p["lds_sh"], self.esize)
+            zp = p.get("zero_pad", 0)
+            if zp:
+                self._check_zero_pad(p, zp)
+            key = (L, tuple(p["rad"]), Tl, p["lds_sh"], self.esize, zp)
             if key not in self.sweeps_checked:
-                emulate_sweep(L, p["rad"], Tl, p["lds_sh"], cod=codelets(self.esize))
+                cod = codelets(self.esize)
+                emulate_sweep(L, p["rad"], Tl, p["lds_sh"], cod=cod)
+                # the first sweep skips the zero tail, the second prunes its outputs
+                if zp & 1:
+                    emulate_sweep(L, p["rad"], Tl, p["lds_sh"], cod=cod, zero_first=True)
+                if zp & 2:
+                    emulate_sweep(L, p["rad"], Tl, p["lds_sh"], cod=cod, prune_last=True)
                 self.sweeps_checked.add(key)
         if p["tw"] is None or p["tw"][0] != TK_STAGE or p["tw"][1] != L:
             raise ScheduleError("stage twiddle table does not match L")
@@ -315,6 +369,13 @@ class Emulator:
         cin = sum(idx[d] * ca[d] for d in range(4))
         cout = sum(idx[d] * cb[d] for d in range(4))
         cmid = sum(idx[d] * p["cm"][d] for d in range(4))
+        cms = np.full_like(cmid, p["cme"])
+        mr = p.get("mirror_rows", 0)
+        if mr:
+            # rows k1 > M1/2 read row M1 - k1 backwards (device.h line_offsets)
+            mir = 2 * idx[0] > mr
+            cmid = np.where(mir, cmid + (mr - 2 * idx[0]) * p["cm"][0] + (L - 1) * p["cme"], cmid)
+            cms = np.where(mir, -p["cme"], p["cme"])
         rader = p["load_mode"] == LOAD_RADER
         if rader:
             if p["rtab"] is None or p["rtab"][0] != TK_RTAB or p["rtab"][1] != L + 1 or p["chirp_n"] != L + 1:
@@ -338,10 +399,13 @@ class Emulator:
             M = p["bhat"][1]
             bhat = bhat_of(p["chirp_n"], M)
             if p["bhat"][0] == TK_BHATROWS:
-                # rows of the fused Bluestein row pass: [k1*M2 + k2] = bhat[k1 + M1*k2], M2 = L
+                # rows 0 .. M1/2 of the fused Bluestein row pass: [k1*M2 + k2] =
+                # bhat[k1 + M1*k2], M2 = L (the others are read mirrored)
                 M2 = L
                 M1 = M // M2
-                k1, k2 = np.divmod(np.arange(M), M2)
+                if p.get("mirror_rows", 0) != M1:
+                    raise ScheduleError("row spectrum without mirrored rows")
+                k1, k2 = np.divmod(np.arange((M1 // 2 + 1) * M2), M2)
                 bhat = bhat[k1 + M1 * k2]
         if p["load_mode"] == 1:
             n = cin[:, None] + t[None, :] * p["cae"]
@@ -376,8 +440,8 @@ class Emulator:
             v = v * np.exp(-2j * np.pi * m / M)
         v = np.fft.fft(v, axis=1)
         if p["mid_mode"] == 1:
-            n = cmid[:, None] + t[None, :] * p["cme"]
-            if bhat is None or n.max() >= bhat.size:
+            n = cmid[:, None] + t[None, :] * cms[:, None]
+            if bhat is None or n.max() >= bhat.size or n.min() < 0:
                 raise ScheduleError("mid spectrum index out of range")
             v = np.conj(v * bhat[n])
             v = np.fft.fft(v, axis=1)

@@ -247,6 +247,63 @@ def test_schedule_batch_uneven():
         off += n
 
 
+# Bluestein lines of 2N-1 <= L points: the first sweep skips the zero tail,
+# the second computes only the positions the chirp store reads
+ZERO_PAD_LENGTHS = [457, 4099, 2 * 7919, 3 * 4099, 1031, 2053]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("n", ZERO_PAD_LENGTHS)
+def test_schedule_zero_padded_bluestein(n, prec):
+    """The planner marks one-kernel Bluestein lines zero-padded (plan.cpp
+    set_zero_pad); the emulator replays both sweeps with the tail slots left
+    as NaN and the last stage's outputs t >= (r + 1) // 2 unwritten, and the
+    whole schedule against numpy's FFT."""
+    s = P.debug_schedule((n,), prec, -1)
+    zp = [la for la in s["launches"] if la.get("zero_pad")]
+    # FP64 runs 2 * 7919 as three fused passes (no one-kernel line)
+    assert zp or (prec == 64 and n == 2 * 7919), [(la["L"], la["rad"], la["load_mode"]) for la in s["launches"]]
+    for la in zp:
+        assert la["chirp_n"] <= la["zero_from"] < la["L"]
+    _check_schedule((n,), prec, -1)
+    _check_schedule((n,), prec, 1)
+
+
+@pytest.mark.parametrize("rad", [[8, 16, 9, 8], [16, 8, 8, 16], [3, 5, 3], [5, 4, 5], [9, 4, 9], [2, 3, 2], [16],
+                                 [4, 5, 4], [3, 16, 3]])
+@pytest.mark.parametrize("Tl", [1, 4, 16, 64])
+def test_emulate_sweep_zero_pad(rad, Tl):
+    L = int(np.prod(rad))
+    # at least the threads every stage's butterflies need
+    Tl = max([Tl] + [-(-(L // r) // max(1, E.EPT // r)) for r in rad])
+    E.emulate_sweep(L, rad, Tl, 0, zero_first=True)
+    E.emulate_sweep(L, rad, Tl, 0, prune_last=True)
+    E.emulate_sweep(L, rad, Tl, 5, zero_first=True, prune_last=True)
+
+
+def test_zero_pad_checker_rejects_wrong_bounds():
+    """A zero tail that starts one row of the first stage too early, or more
+    inputs than the bound covers, or a radix without the variant, is refused."""
+    s = P.debug_schedule((4099,), 32, -1)
+    la = next(la for la in s["launches"] if la.get("zero_pad"))
+    E.Emulator._check_zero_pad(la, la["zero_pad"])
+    r0 = la["rad"][0]
+    bad = dict(la, zero_from=la["zero_from"] - la["L"] // r0)
+    with pytest.raises(E.ScheduleError):
+        E.Emulator._check_zero_pad(bad, 1)
+    bad = dict(la, chirp_n=la["zero_from"] + 1)
+    with pytest.raises(E.ScheduleError):
+        E.Emulator._check_zero_pad(bad, 1)
+    with pytest.raises(E.ScheduleError):
+        E.Emulator._check_zero_pad(bad, 2)
+    bad = dict(la, rad=[7] + la["rad"][1:], zero_from=4 * (la["L"] // 7))
+    with pytest.raises(E.ScheduleError):
+        E.Emulator._check_zero_pad(bad, 1)
+    bad = dict(la, store_mode=1)
+    with pytest.raises(E.ScheduleError):
+        E.Emulator._check_zero_pad(bad, 3)
+
+
 DIRECT_PRIME_LENGTHS = [17, 19, 23 * 16, 31 * 27, 37 * 64, 41 * 43, 47 * 128, 53 * 35, 59 * 3, 61 * 61, 67 * 2,
                         71 * 256, 71 * 71 * 4, 17 * 4096, 29 * 31 * 37, 43 * 1024, 67 * 67]
 
