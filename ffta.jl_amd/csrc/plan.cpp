@@ -1881,6 +1881,8 @@ struct fftahip_batch_s {
     int ngroups;
     size_t pass_off, tile_off;  // element offsets into passes / tiles
     std::vector<int> deps;      // earlier launches holding the previous pass of one of its groups
+    int pass = 0;               // which pass of its groups the launch runs
+    std::vector<int> members;   // its groups, in tile order (dry-run inspection)
   };
   std::vector<Phase> phases;
   std::vector<LinePass> host_passes;
@@ -2793,11 +2795,13 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
           b->host_tiles.push_back(tiles);
           tiles += (uint32_t)L[ph].grid;
           ++P.ngroups;
+          if (g_dry) P.members.push_back((int)g);
           add_deps(P, g, ph);
           launch_of[g][ph] = (int)b->phases.size();
         }
         if (P.ngroups == 0) continue;
         P.grid = tiles;
+        P.pass = (int)ph;
         b->phases.push_back(P);
       }
     };
@@ -2829,11 +2833,13 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
               tiles += (uint32_t)L[ph].grid;
               P.lds = std::max(P.lds, L[ph].lds);
               ++P.ngroups;
+              if (g_dry) P.members.push_back((int)g);
               add_deps(P, g, ph);
               launch_of[g][ph] = (int)b->phases.size();
             }
             if (P.ngroups == 0) continue;
             P.grid = tiles;
+            P.pass = (int)ph;
             b->phases.push_back(P);
           }
     };
@@ -3158,8 +3164,11 @@ fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const 
   for (size_t i = 0; i < b->phases.size(); ++i) {
     const auto& P = b->phases[i];
     os << (i ? ", " : "") << "{\"reg\": " << (P.reg ? 1 : 0) << ", \"block\": " << P.block << ", \"mode\": " << P.mode
-       << ", \"lds\": " << P.lds << ", \"grid\": " << P.grid << ", \"groups\": " << P.ngroups << ", \"deps\": [";
+       << ", \"lds\": " << P.lds << ", \"grid\": " << P.grid << ", \"groups\": " << P.ngroups
+       << ", \"pass\": " << P.pass << ", \"deps\": [";
     for (size_t d = 0; d < P.deps.size(); ++d) os << (d ? ", " : "") << P.deps[d];
+    os << "], \"members\": [";
+    for (size_t m = 0; m < P.members.size(); ++m) os << (m ? ", " : "") << P.members[m];
     os << "]}";
   }
   os << "]}";
