@@ -257,10 +257,14 @@ __device__ __forceinline__ void stage(cx<Rt>* line, int jl, bool active, int Tl,
 // A work item is (butterfly j, group g of kBigKP consecutive k); butterflies
 // run fastest across threads, and a thread holds at most kBigHold work
 // items across the barrier (the planner sizes Tl for that).  `roots` holds
-// w_P^m = exp(-2 pi i m/P), m < P, in the pass precision, staged in LDS
-// (the tile copies them there before its first barrier); the exponent j*k
-// mod P advances by k per step.  Stage twiddles (Ns > 1) are applied to the
-// inputs as they are read, from the same per-stage table as stage().
+// w_P^(m mod P) = exp(-2 pi i m/P) for m < kBigKP * P (the P roots repeated),
+// in the pass precision, staged in LDS (the tile copies them there before its
+// first barrier).  For the work item's k = k0 + u, the exponent i*k mod P is
+// (i*k0 mod P) + u*i: one reduction per step for all kBigKP roots, and the
+// repeated table absorbs the rest.  Positions j + i*L/P whose padding is
+// linear in i (L/P a multiple of the padding block) are stepped.  Stage
+// twiddles (Ns > 1) are applied to the inputs as they are read, from the same
+// per-stage table as stage().
 constexpr int kBigKP = 4;
 constexpr int kBigHold = 2;
 
@@ -273,6 +277,9 @@ __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int
   const int G = (H + kBigKP - 1) / kBigKP;
   const int nwork = Lr * G;
   const int she = pad_shift(sh);
+  const int pmask = sh ? (1 << sh) - 1 : 0;
+  const bool rstep = (Lr & pmask) == 0;
+  const int rstride = Lr + (sh ? (Lr >> sh) : 0);
   cx<Rt> out[kBigHold][2 * kBigKP + 1];
   int obase[kBigHold], og[kBigHold];
 #pragma unroll
@@ -286,34 +293,41 @@ __device__ __forceinline__ void stage_big(cx<Rt>* line, int jl, bool active, int
     const uint32_t q = fdiv((uint32_t)j, nsd);
     const int k = j - (int)q * Ns;
     const cx<Rt>* __restrict__ wt = tw + (Ns - 1) + k * (P - 1);
-    const cx<Rt> x0 = line[pad_idx(j, she)];
+    const int pj = pad_idx(j, she);
+    const cx<Rt> x0 = line[pj];
     cx<Rt> A[kBigKP], B[kBigKP];
-    int m[kBigKP];
 #pragma unroll
     for (int u = 0; u < kBigKP; ++u) {
       A[u] = x0;
       B[u] = {(Rt)0, (Rt)0};
-      m[u] = 0;
     }
     cx<Rt> sum = x0;
     const int k0 = g * kBigKP + 1;
+    int mb = 0;  // i * k0 mod P
     // two steps in flight for FP32; FP64 has no registers for a second
     constexpr int kUnroll = sizeof(Rt) == 4 ? 2 : 1;
 #pragma unroll kUnroll
     for (int i = 1; i <= H; ++i) {
-      cx<Rt> a = line[pad_idx(j + i * Lr, she)];
-      cx<Rt> b = line[pad_idx(j + (P - i) * Lr, she)];
+      cx<Rt> a, b;
+      if (rstep) {
+        a = line[pj + i * rstride];
+        b = line[pj + (P - i) * rstride];
+      } else {
+        a = line[pad_idx(j + i * Lr, she)];
+        b = line[pad_idx(j + (P - i) * Lr, she)];
+      }
       if (Ns > 1) {
         a = cmul(a, gld(wt + (i - 1)));
         b = cmul(b, gld(wt + (P - i - 1)));
       }
       const cx<Rt> s = cadd(a, b), d = csub(a, b);
       sum = cadd(sum, s);
+      mb += k0;  // < 2P: the unsigned minimum subtracts P when mb >= P
+      mb = (int)min((unsigned)mb, (unsigned)(mb - P));
+      const cx<Rt>* __restrict__ rr = roots + mb;
 #pragma unroll
       for (int u = 0; u < kBigKP; ++u) {
-        m[u] += k0 + u;
-        if (m[u] >= P) m[u] -= P;
-        const cx<Rt> r = roots[m[u]];  // (cos, -sin) of 2 pi m / P
+        const cx<Rt> r = rr[u * i];  // (cos, -sin) of 2 pi (i (k0 + u)) / P
         A[u].x += r.x * s.x;
         A[u].y += r.x * s.y;
         B[u].x -= r.y * d.x;
@@ -437,10 +451,11 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
   const int sh = p.lds_sh;
   const bool ws = p.wave_sync != 0;
   const int nst = s_end < 0 ? p.nst : s_end;
-  // roots of the radix-P stages with P >= 17, one block of P per stage (LDS)
+  // roots of the radix-P stages with P >= 17, one block of kBigKP * P per
+  // stage (LDS)
   int roots_off = 0;
   for (int s = 0; s < s_begin; ++s) {
-    if (BIG && big_stage(p.rad[s], kPrec<Rt>)) roots_off += p.rad[s];
+    if (BIG && big_stage(p.rad[s], kPrec<Rt>)) roots_off += kBigKP * p.rad[s];
     Ns *= p.rad[s];
   }
   for (int s = s_begin; s < nst; ++s) {
@@ -452,7 +467,7 @@ __device__ __forceinline__ void sweep(cx<Rt>* line, int jl, bool active, const L
       // the planner keeps fused global access and the in-stage spectrum
       // product off stages of this kind
       stage_big<Rt>(line, jl, active, p.Tl, p.L, r, Ns, nsd, tw, roots + roots_off, sh, ws);
-      roots_off += r;
+      roots_off += kBigKP * r;
       Ns *= r;
       continue;
     }
@@ -650,15 +665,20 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   const int nvalid = rem < C ? (int)rem : C;
 
   // direct-DFT roots (BIG variants): the stage table's entries after its
-  // L - 1 stage twiddles, staged after the line tables
+  // L - 1 stage twiddles, P per direct-DFT stage, staged after the line
+  // tables as kBigKP * P entries per stage (the P roots repeated)
   cx<Rt>* roots = nullptr;
   if (BIG) {
     roots = reinterpret_cast<cx<Rt>*>(reinterpret_cast<unsigned char*>(lds) + tile_lds_bytes(C, ldl, sizeof(cx<Rt>)));
-    int nroots = 0;
-    for (int s = 0; s < p.nst; ++s)
-      if (big_stage(p.rad[s], kPrec<Rt>)) nroots += p.rad[s];
     const cx<Rt>* __restrict__ src = reinterpret_cast<const cx<Rt>*>(p.tw) + (L - 1);
-    for (int i = tid; i < nroots; i += BLOCK) roots[i] = gld(src + i);
+    int lo = 0;
+    for (int s = 0; s < p.nst; ++s) {
+      const int P = p.rad[s];
+      if (!big_stage(P, kPrec<Rt>)) continue;
+      for (int i = tid; i < kBigKP * P; i += BLOCK) roots[lo + i] = gld(src + i % P);
+      lo += kBigKP * P;
+      src += P;
+    }
   }
   for (int c = tid; c < nvalid; c += BLOCK) {
     const LineOffsets o = line_offsets<MODE>(p, (uint32_t)(line0 + c));

@@ -238,6 +238,8 @@ constexpr int64_t kMaxDirectRoots = 760;
 // Direct-DFT stage geometry (device.h): work items of kBigKP output pairs,
 // kBigHold per thread.
 constexpr int64_t kBigKPHost = 4, kBigHoldHost = 2;
+static int64_t lds_budget();
+static size_t csize(int prec);
 
 // In-LDS line schedule with the given radices (in stage order): threads per
 // line Tl = max over stages of ceil((L / R) / (16 / R)) for codelet radices,
@@ -261,6 +263,13 @@ static bool inner_from(int64_t L, const std::vector<int>& rad, Inner& in, int pr
   for (int r : rad)
     if (!codelet_radix(r, prec)) roots += r;
   if (prod != L || Tl > 1024 || roots > kMaxDirectRoots) return false;
+  // one unpadded line and its staged roots (kBigKP copies per direct-DFT
+  // stage) must fit one workgroup's LDS; the pass then takes a padding that
+  // still fits (set_inner).  prec 0, the planner's precision-free probes:
+  // sized as FP64, so that a probe never admits a schedule the pass's own
+  // precision rejects.
+  const int64_t esz_fit = (int64_t)csize(prec == 32 ? 32 : 64);
+  if (roots > 0 && (L + 1 + kBigKPHost * roots) * esz_fit > lds_budget()) return false;
   in = Inner();
   in.L = L;
   in.rad = rad;
@@ -729,7 +738,8 @@ static fftahip_status make_decomp_rule(LengthPlan& lp, int strided, Decomp& d) {
     d.inner = {qb};
     for (int64_t f : FS) {
       Inner in;
-      make_inner(f, in, prec, lp.cutoff);
+      if (!make_inner(f, in, prec, lp.cutoff))
+        return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no line schedule for pass length " + std::to_string(f));
       d.F.push_back(f);
       d.inner.push_back(in);
     }
@@ -764,8 +774,8 @@ static fftahip_status make_decomp_rule(LengthPlan& lp, int strided, Decomp& d) {
     d.kind = K_BLUE_3PASS;
     d.F = {M1, M2};
     Inner a, b;
-    make_inner(M1, a, prec);
-    make_inner(M2, b, prec);
+    if (!make_inner(M1, a, prec) || !make_inner(M2, b, prec))
+      return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no line schedule for the Bluestein passes");
     d.inner = {a, b};
     return FFTAHIP_SUCCESS;
   }
@@ -776,7 +786,8 @@ static fftahip_status make_decomp_rule(LengthPlan& lp, int strided, Decomp& d) {
   d.F = F;
   for (int64_t f : F) {
     Inner in;
-    make_inner(f, in, prec, cut);
+    if (!make_inner(f, in, prec, cut))
+      return fail(FFTAHIP_ERR_NOT_IMPLEMENTED, "no line schedule for pass length " + std::to_string(f));
     d.inner.push_back(in);
   }
   return FFTAHIP_SUCCESS;
@@ -1001,10 +1012,11 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
   const int64_t padmax = in.L + (in.L - 1) / 16 + 1 + side;  // largest pitch any padding choice needs
   const int64_t pbud = in.tile_budget > 0 ? std::min(in.tile_budget, lds_budget()) : pass_budget();
   const int64_t budget = strided ? std::min(lds_budget(), std::max<int64_t>(pbud, padmax * esz)) : lds_budget();
-  // direct-DFT stages stage their roots after the tile's line tables
+  // direct-DFT stages stage their roots after the tile's line tables, each
+  // stage's P roots repeated kBigKP times
   int64_t roots_bytes = 0;
   for (int r : in.rad)
-    if (!codelet_radix(r, prec)) roots_bytes += r * esz;
+    if (!codelet_radix(r, prec)) roots_bytes += kBigKPHost * r * esz;
   while (C > 1 && C * padmax * esz + roots_bytes > budget) --C;
   if (C > p.nlines) C = std::max<int64_t>(1, p.nlines);
   // LDS padding: env FFTAHIP_LDS_PAD (-1 = choose by the bank model)
@@ -1034,6 +1046,8 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
     for (int extra : {0, 1}) {
       if (C == 1 && extra) continue;
       const int64_t pitch = padded((int)in.L - 1, sh) + 1 + extra + side;
+      // a padding the tile (lines and staged roots) no longer fits is out
+      if (sh && C * pitch * esz + roots_bytes > lds_budget()) continue;
       const double cst = lds_cost(inw, (int)C, (int)pitch, sh, words, p.col_in, p.col_out);
       if (cst < best - 1e-9) {
         best = cst;
@@ -1043,7 +1057,8 @@ static void set_inner(LinePass& p, const Inner& in, int prec, Launch& L, bool al
     }
   }
   if (force > 0 && best > 1e29) {
-    best_sh = (int)force;
+    // the forced padding (when the tile still fits it), else none
+    best_sh = C * (padded((int)in.L - 1, (int)force) + 1 + side) * esz + roots_bytes <= lds_budget() ? (int)force : 0;
     ldl = padded((int)in.L - 1, best_sh) + 1 + side;
   }
   {

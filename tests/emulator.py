@@ -112,6 +112,14 @@ def _big_stage(lds, tw, L, R, Ns, Tl, sh, roots_off, P):
     roots = tw[roots_off:roots_off + R]
     if not np.allclose(roots, np.exp(-2j * np.pi * np.arange(R) / R), rtol=0, atol=1e-15):
         raise ScheduleError("direct-DFT roots wrong")
+    # the kernel reads root (i*k0 mod R) + u*i for k = k0 + u from the R roots
+    # repeated BIG_KP times in LDS: in range, and the root of i*k mod R
+    i = np.arange(1, H + 1)[:, None, None]
+    k0 = (np.arange(G) * BIG_KP + 1)[None, :, None]
+    u = np.arange(BIG_KP)[None, None, :]
+    ridx = (i * k0) % R + u * i
+    if ridx.max() >= BIG_KP * R or np.any(ridx % R != (i * (k0 + u)) % R):
+        raise ScheduleError(f"direct radix {R}: root index beyond the staged table")
     new = np.full(P, np.nan + 0j)
     hits = np.zeros(P, dtype=np.int64)
     t = np.arange(R)
@@ -324,7 +332,8 @@ class Emulator:
                 raise ScheduleError(f"wave_sync with {Tl} threads per line")
             sh = p["lds_sh"]
             need = L + ((L - 1) >> sh if sh else 0) + (RADER_SIDE if p["load_mode"] == LOAD_RADER else 0)
-            roots = sum(r for r in p["rad"] if r not in codelets(self.esize)) * self.esize
+            # each direct-DFT stage stages its P roots repeated BIG_KP times
+            roots = sum(BIG_KP * r for r in p["rad"] if r not in codelets(self.esize)) * self.esize
             if p["ldl"] < need or p["lds"] != ((C * p["ldl"] * self.esize + 15) & ~15) + C * LINE_TAB_BYTES + roots:
                 raise ScheduleError("LDS size mismatch")
             if p["lds"] > lds_per_block:
