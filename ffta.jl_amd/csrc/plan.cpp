@@ -641,6 +641,9 @@ static bool blue3_split(int64_t M, int prec, int64_t* M1, int64_t* M2) {
 static bool split_factors(int64_t M, int64_t cap, std::vector<int64_t>& F, int64_t cutoff = kInternalCutoff) {
   std::vector<int64_t> primes = prime_factors(M);
   std::sort(primes.begin(), primes.end(), std::greater<int64_t>());
+  // a direct prime above the cap gets a pass of its own (a strided pass then
+  // holds fewer adjacent lines per tile)
+  if (!primes.empty()) cap = std::max(cap, primes.front());
   for (int p = 2; p <= 8; ++p) {
     std::vector<int64_t> g(p, 1);
     bool ok = true;
@@ -3133,11 +3136,12 @@ extern "C" {
 // Schedule of mul!(y, plan_fft(x, region), x) for an array of shape dims on a
 // device with `lds_per_block` bytes of LDS, as JSON: {"scratch_bytes", "launches"}.
 fftahip_status fftahip_debug_schedule(int precision, int direction, int ndims, const int64_t* dims, int nregion,
-                                      const int64_t* region, int64_t lds_per_block, char* buf, size_t cap,
-                                      size_t* needed) {
+                                      const int64_t* region, int64_t bluestein_cutoff, int64_t lds_per_block,
+                                      char* buf, size_t cap, size_t* needed) {
   DryGuard g(lds_per_block);
   fftahip_plan plan = nullptr;
-  fftahip_status st = fftahip_plan_create(&plan, precision, direction, ndims, dims, nregion, region, 73);
+  fftahip_status st =
+      fftahip_plan_create(&plan, precision, direction, ndims, dims, nregion, region, bluestein_cutoff);
   if (st) return st;
   std::unique_ptr<fftahip_plan_s> owner(plan);
   std::vector<Launch> L;

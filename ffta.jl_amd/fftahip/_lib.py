@@ -86,7 +86,7 @@ SIGNATURES = {
     "fftahip_synchronize": (_st, []),
     "fftahip_device_info": (_st, [_i64p, _i64p, _i64p]),
     "fftahip_debug_schedule": (_st, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int, _i64p,
-                                     ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t,
+                                     ctypes.c_int64, ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t,
                                      ctypes.POINTER(ctypes.c_size_t)]),
     "fftahip_debug_batch_schedule": (_st, [ctypes.c_int, ctypes.c_int64, _i64p, ctypes.POINTER(ctypes.c_int),
                                            ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t,

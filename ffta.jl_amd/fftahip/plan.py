@@ -383,12 +383,14 @@ class Batch:
         return json.loads(txt)
 
 
-def debug_schedule(shape, precision: int, direction: int, region=None, lds_per_block: int = 160 * 1024) -> dict:
-    """Dry-run schedule of mul!(y, plan, x) for an array of `shape` (no device needed)."""
+def debug_schedule(shape, precision: int, direction: int, region=None, lds_per_block: int = 160 * 1024,
+                   cutoff: int = 73) -> dict:
+    """Dry-run schedule of mul!(y, plan, x) for an array of `shape` (no device
+    needed), planned with BLUESTEIN_CUTOFF `cutoff`."""
     dims = tuple(int(d) for d in shape)
     reg = _region_tuple(region, len(dims))
     st, txt = _lib.read_json(load().fftahip_debug_schedule, precision, direction, len(dims), i64_array(dims),
-                             len(reg), i64_array(reg), lds_per_block)
+                             len(reg), i64_array(reg), int(cutoff), lds_per_block)
     _check(st)
     return json.loads(txt)
 

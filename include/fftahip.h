@@ -265,11 +265,12 @@ fftahip_status fftahip_device_info(int64_t* lds_per_block, int64_t* cu_count, in
  * Dry runs of the planner (no device needed): the launches that
  * fftahip_execute / fftahip_batch_execute would issue on a device with
  * `lds_per_block` bytes of LDS, as JSON with device pointers replaced by
- * [kind, offset] tokens.  No reference counterpart; used by the test-suite to
- * check pass geometry on the host. */
+ * [kind, offset] tokens (fftahip_debug_schedule plans with the given
+ * BLUESTEIN_CUTOFF, as fftahip_plan_create does).  No reference counterpart;
+ * used by the test-suite to check pass geometry on the host. */
 fftahip_status fftahip_debug_schedule(int precision, int direction, int ndims, const int64_t* dims, int nregion,
-                                      const int64_t* region, int64_t lds_per_block, char* buf, size_t cap,
-                                      size_t* needed);
+                                      const int64_t* region, int64_t bluestein_cutoff, int64_t lds_per_block,
+                                      char* buf, size_t cap, size_t* needed);
 fftahip_status fftahip_debug_batch_schedule(int precision, int64_t count, const int64_t* lengths,
                                             const int* directions, int64_t lds_per_block, char* buf, size_t cap,
                                             size_t* needed);

@@ -185,6 +185,9 @@ CUTOFF_CASES = [
     (5, 2), (5, 7), (13, 11), (13, 73), (17, 17), (17, 18), (71, 73), (71, 71), (101, 73), (101, 200),
     (199 * 3, 200), (199 * 3, 199), (19 * 16 * 7, 19), (19 * 16 * 7, 20), (997, 1000), (1009 * 2, 2000), (743, 800), (743 * 4, 744),
     (73 * 73, 73), (73 * 73, 74), (4096, 2),
+    # a direct prime above the strided-pass cap, and in FP64 a line too long
+    # for one workgroup with its staged roots: a pass of its own
+    (743 * 12, 752), (751 * 8, 752),
 ]
 
 

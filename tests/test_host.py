@@ -199,8 +199,8 @@ SHAPES_1D = [1, 2, 3, 16, 17, 64, 100, 103, 243, 1000, 1009, 2187, 3000, 4096, 5
              30000, 65536, 131072, 177147, 262144]
 
 
-def _check_schedule(shape, prec, direction, region=None, lds=160 * 1024):
-    s = P.debug_schedule(shape, prec, direction, region, lds)
+def _check_schedule(shape, prec, direction, region=None, lds=160 * 1024, cutoff=73):
+    s = P.debug_schedule(shape, prec, direction, region, lds, cutoff)
     rng = np.random.default_rng(sum(shape))
     x = np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
     y = E.run_plan_schedule(s, x, 8 if prec == 32 else 16, lds)
@@ -302,6 +302,22 @@ def test_zero_pad_checker_rejects_wrong_bounds():
     bad = dict(la, store_mode=1)
     with pytest.raises(E.ScheduleError):
         E.Emulator._check_zero_pad(bad, 3)
+
+
+# primes up to 751 as direct-DFT stages (BLUESTEIN_CUTOFF 752): long lines
+# whose staged roots (four copies per stage) leave no room for one workgroup
+# split into more passes; every schedule still emulates exactly
+LARGE_DIRECT_CASES = [743, 743 * 4, 743 * 12, 751 * 8, 701 * 16, 509 * 15, 653 * 9, 599 * 2 * 3, 743 * 743]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("n", LARGE_DIRECT_CASES)
+def test_schedule_large_direct_primes(n, prec):
+    s = P.debug_schedule((n,), prec, -1, None, 160 * 1024, 752)
+    assert all(la["load_mode"] == 0 for la in s["launches"]), [(la["L"], la["rad"]) for la in s["launches"]]
+    for la in s["launches"]:
+        assert la["lds"] <= 160 * 1024
+    _check_schedule((n,), prec, -1 if n % 2 else 1, None, 160 * 1024, 752)
 
 
 DIRECT_PRIME_LENGTHS = [17, 19, 23 * 16, 31 * 27, 37 * 64, 41 * 43, 47 * 128, 53 * 35, 59 * 3, 61 * 61, 67 * 2,
