@@ -158,6 +158,36 @@ def run_serving(torch, srv, rate: float = 50000.0, seconds: float = 0.5, seed: i
     return st
 
 
+def device_report(torch, stream, mib: int = 1024, reps: int = 10) -> dict:
+    """The device as the runtime reports it (name, CUs, LDS per workgroup,
+    memory) and the HBM rate of a plain device copy of `mib` MiB (read +
+    write bytes over the time of one copy, events on `stream`): the practical
+    ceiling that the roofline's datasheet peak stands beside."""
+    from fftahip import device_info
+    prop = torch.cuda.get_device_properties(0)
+    info = device_info()
+    lds, cus, warp = info["lds_per_block"], info["cu_count"], info["warp_size"]
+    n = mib * (1 << 20) // 8
+    a = torch.empty(n, dtype=torch.complex64, device="cuda")
+    a.zero_()
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        b.copy_(a)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / reps
+    del a, b
+    copy_gbs = 2.0 * n * 8 / (ms * 1e-3) / 1e9
+    return {"name": prop.name, "compute_units": int(cus), "lds_per_workgroup": int(lds), "wavefront": int(warp),
+            "hbm_bytes": int(prop.total_memory), "copy_GBps": round(copy_gbs, 1),
+            "copy_frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 3), "copy": f"torch device copy of {mib} MiB"}
+
+
 def flops_of(lengths) -> float:
     L = np.asarray(lengths, dtype=np.float64)
     return float(np.sum(np.where(L > 1, 5.0 * L * np.log2(np.maximum(L, 1)), 0.0)))
@@ -512,6 +542,7 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         del a, b, kb
 
+    device = device_report(torch, stream)
     flops = flops_of(lengths)
     bytes_step = 2.0 * total * 8
     achieved = bytes_step / (ms * 1e-3) / 1e9
@@ -554,6 +585,7 @@ def main(argv=None) -> int:
         "error": {"worst_relerr_over_bound": round(worst, 4), "sample": nsample,
                   "bound": "2e-6*log2(N) vs FP64 oracle"},
         "plan_seconds": round(plan_s, 2),
+        "device": device,
     }
     if args.serve_seconds > 0:
         from fftahip import Server
