@@ -1274,22 +1274,25 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
     // a, st < N (the planner attaches chirp steps only then): exact reductions
     double2 c = chirp_root_r((int64_t)mod_q31(a * a, 2 * (uint64_t)N, 0.5 * p.chirp_inv), N);
     const double2 w = chirp_root_r((int64_t)(2 * mod_q31(a * st, (uint64_t)N, p.chirp_inv)), N);
-    // element loads in chunks of kChirpChunk in flight (register budget)
+    // element loads in chunks of kChirpChunk in flight (register budget);
+    // the live positions are a prefix t < tl, and the dead ones (zero in a
+    // padded Bluestein line, about half of them) load and compute nothing
+    const int tl = live_bound(N - o.cin, p.cae, L);
     constexpr int kChirpChunk = 8;
 #pragma unroll
     for (int t0 = 0; t0 < L; t0 += kChirpChunk) {
 #pragma unroll
       for (int t = t0; t < t0 + kChirpChunk && t < L; ++t) {
-        const bool live = o.cin + (int64_t)t * p.cae < N;
-        v[t] = gld(in + (live ? (int64_t)t * p.ae : 0));
+        v[t] = {(Rt)0, (Rt)0};
+        if (t < tl) v[t] = gld(in + (int64_t)t * p.ae);
       }
 #pragma unroll
       for (int t = t0; t < t0 + kChirpChunk && t < L; ++t) {
-        const bool live = o.cin + (int64_t)t * p.cae < N;
-        const cx<Rt> ct = from_d2<Rt>(dmul2(c, uld(p.chirp_d + t)));
-        const cx<Rt> y = cmul(cconj_if(v[t], sgn_in), cconj(ct));
-        v[t] = live ? y : cx<Rt>{(Rt)0, (Rt)0};
-        c = dmul2(c, w);
+        if (t < tl) {
+          const cx<Rt> ct = from_d2<Rt>(dmul2(c, uld(p.chirp_d + t)));
+          v[t] = cmul(cconj_if(v[t], sgn_in), cconj(ct));
+          c = dmul2(c, w);
+        }
       }
       __builtin_amdgcn_sched_barrier(0);
     }
