@@ -1346,14 +1346,18 @@ __device__ __forceinline__ void reg_line(const LinePass& p, const BufPtrs& bp, u
     const uint64_t Nc = (uint64_t)p.chirp_n;
     double2 oc = chirp_root_r((int64_t)mod_q31(a * a, 2 * Nc, 0.5 * p.chirp_inv), p.chirp_n);
     const double2 ow = chirp_root_r((int64_t)(2 * mod_q31(a * st, Nc, p.chirp_inv)), p.chirp_n);
+    // the stored positions are a prefix k < kl (the rest of a padded
+    // Bluestein column is dropped): no chirp work past it
+    const int kl = live_bound(p.chirp_n - o.cout, p.cbe, L);
 #pragma unroll
     for (int k = 0; k < L; ++k) {
-      const bool live = o.cout + (int64_t)k * p.cbe < p.chirp_n;
-      cx<Rt> y = cconj(cmul(from_d2<Rt>(dmul2(oc, uld(p.chirp_d + k))), v[k]));
-      oc = dmul2(oc, ow);
-      y = cconj_if(y, sgn_out);
-      if (has_scale) y = cscale(y, scale);
-      if (live) gst(out + (int64_t)k * p.be, y);
+      if (k < kl) {
+        cx<Rt> y = cconj(cmul(from_d2<Rt>(dmul2(oc, uld(p.chirp_d + k))), v[k]));
+        oc = dmul2(oc, ow);
+        y = cconj_if(y, sgn_out);
+        if (has_scale) y = cscale(y, scale);
+        gst(out + (int64_t)k * p.be, y);
+      }
       if (k % kRegChunk == kRegChunk - 1) __builtin_amdgcn_sched_barrier(0);
     }
     return;
@@ -1413,9 +1417,15 @@ __device__ __forceinline__ void reg_dispatch(const LinePass& p, const BufPtrs& b
 }
 
 // Work-list launch of register passes: tile = 256 consecutive lines of one
-// group (pass); tiles in the XCD-aware order of line_pass_group_kernel.
+// group (pass); tiles in the XCD-aware order of line_pass_group_kernel.  The
+// FP32 full variant (the three-pass Bluestein column passes, latency-bound)
+// is built for 6 resident waves per SIMD (80 VGPRs, 8 bytes of spill in its
+// longest lines): more loads in flight per CU; the batched step ran 3.36 ms
+// against 3.46 ms with 4 waves and 3.39 ms with 5, while 8 waves (64 VGPRs)
+// spill 136 bytes and ran 3.51 ms.
 template <class Rt, int MODE>
-__global__ void __launch_bounds__(kRegBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kRegBlock)
+__attribute__((amdgpu_waves_per_eu(sizeof(Rt) == 4 && MODE == MODE_FULL ? 6 : 4)))
 reg_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __restrict__ tile_start, int ngroups,
                       BufPtrs bp, uint32_t ntiles) {
   const uint32_t bid = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
