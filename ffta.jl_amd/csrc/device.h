@@ -1202,8 +1202,10 @@ __device__ __forceinline__ void run_line_pass(const LinePass& p, const BufPtrs& 
   }
 }
 
-// Work-list launch: group g owns tiles [tile_start[g], tile_start[g+1]) of
-// ntiles.  Every group is an independent LinePass of the same precision,
+// Work-list launch of ntiles tiles over ngroups groups: with several groups,
+// tile_start holds two words per tile, its group and its tile index within
+// the group (one pair of loads per workgroup instead of a search); with one
+// group it is not read.  Every group is an independent LinePass of the same precision,
 // block size and mode; groups may differ in length, radices and fused
 // operations.  Tiles are visited in XCD-aware order (workgroups are dealt
 // round-robin to the 8 XCDs, so tile = (bid % 8) * (grid / 8) + bid / 8 with
@@ -1222,17 +1224,18 @@ line_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __re
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const uint32_t bid = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   if (bid >= ntiles) return;
-  int lo = 0, hi = ngroups - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tile_start[mid] <= bid) lo = mid;
-    else hi = mid - 1;
+  // several groups: tile_start holds (group, tile within the group) per tile
+  int lo = 0;
+  uint32_t local = bid;
+  if (ngroups > 1) {
+    lo = (int)tile_start[2 * bid];
+    local = tile_start[2 * bid + 1];
   }
   const LinePass& p = passes[lo];
   const int C = p.C;
   int64_t* tab = reinterpret_cast<int64_t*>(smem_raw + (tile_lds_bytes(C, p.ldl, sizeof(cx<Rt>)) - (size_t)C * kLineTabBytes));
   const LineShared sh{tab, tab + C, tab + 2 * C, tab + 3 * C, tab + 4 * C, tab + 5 * C};
-  run_line_pass<Rt, EPT, BLOCK, MODE, BIG>(p, bp, bid - tile_start[lo], reinterpret_cast<cx<Rt>*>(smem_raw), sh);
+  run_line_pass<Rt, EPT, BLOCK, MODE, BIG>(p, bp, local, reinterpret_cast<cx<Rt>*>(smem_raw), sh);
 }
 
 // ------------------------------------------------------------------ register line passes
@@ -1430,14 +1433,15 @@ reg_pass_group_kernel(const LinePass* __restrict__ passes, const uint32_t* __res
                       BufPtrs bp, uint32_t ntiles) {
   const uint32_t bid = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   if (bid >= ntiles) return;
-  int lo = 0, hi = ngroups - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tile_start[mid] <= bid) lo = mid;
-    else hi = mid - 1;
+  // several groups: tile_start holds (group, tile within the group) per tile
+  int lo = 0;
+  uint32_t local = bid;
+  if (ngroups > 1) {
+    lo = (int)tile_start[2 * bid];
+    local = tile_start[2 * bid + 1];
   }
   const LinePass& p = passes[lo];
-  const int64_t line = (int64_t)(bid - tile_start[lo]) * kRegBlock + threadIdx.x;
+  const int64_t line = (int64_t)local * kRegBlock + threadIdx.x;
   if (line >= p.nlines) return;
   reg_dispatch<Rt, MODE>(p, bp, (uint32_t)line);
 }

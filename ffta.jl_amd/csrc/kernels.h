@@ -48,7 +48,9 @@ constexpr int kBigVariant = 3;
 constexpr int kLdsVariants = 6;
 int pass_variant(const LinePass& p);
 // Launch ngroups independent passes (device descriptors) as one grid.
-// `variant`: pass_variant of every group of the launch.
+// `variant`: pass_variant of every group of the launch.  d_tile_start: with
+// ngroups > 1, (group, tile within the group) for each of the grid's tiles;
+// unused with one group.
 hipError_t launch_line_pass_groups(const LinePass* d_passes, const uint32_t* d_tile_start, int ngroups, int variant,
                                    int precision, int block, const BufPtrs& bp, int remap, size_t lds_bytes,
                                    int64_t grid, hipStream_t stream);

@@ -2792,6 +2792,24 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
   // 3.62 ms per step eager, 3.49 vs 3.56 ms as a graph, against register
   // launches first and blocks 256 -> 1024: env FFTAHIP_BATCH_ORDER=0).
   const bool old_order = env_i64("FFTAHIP_BATCH_ORDER", 1) == 0;
+  // a launch's tile table in host_tiles (its offset returned): one unused
+  // word for a single group, else (group, tile within the group) per tile,
+  // so that a workgroup finds its pass with one pair of loads
+  auto tile_map = [&](const std::vector<uint32_t>& starts, uint32_t ntiles) -> size_t {
+    const size_t off = b->host_tiles.size();
+    if (starts.size() <= 1) {
+      b->host_tiles.push_back(0);
+      return off;
+    }
+    for (size_t g = 0; g < starts.size(); ++g) {
+      const uint32_t end = g + 1 < starts.size() ? starts[g + 1] : ntiles;
+      for (uint32_t t = starts[g]; t < end; ++t) {
+        b->host_tiles.push_back((uint32_t)g);
+        b->host_tiles.push_back(t - starts[g]);
+      }
+    }
+    return off;
+  };
   for (size_t ph = 0; ph < maxp; ++ph) {
     auto reg_launches = [&](int mode_hi, int mode_lo) {
       // register passes of this phase: one launch per mode, the full
@@ -2804,13 +2822,13 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
         P.lds = 0;
         P.ngroups = 0;
         P.pass_off = b->host_passes.size();
-        P.tile_off = b->host_tiles.size();
+        std::vector<uint32_t> starts;
         uint32_t tiles = 0;
         for (size_t g = 0; g < per_group.size(); ++g) {
           auto& L = per_group[g];
           if (ph >= L.size() || !L[ph].reg || L[ph].grid <= 0 || pass_mode(L[ph].lp) != mode) continue;
           b->host_passes.push_back(L[ph].lp);
-          b->host_tiles.push_back(tiles);
+          starts.push_back(tiles);
           tiles += (uint32_t)L[ph].grid;
           ++P.ngroups;
           if (g_dry) P.members.push_back((int)g);
@@ -2820,6 +2838,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
         if (P.ngroups == 0) continue;
         P.grid = tiles;
         P.pass = (int)ph;
+        P.tile_off = tile_map(starts, tiles);
         b->phases.push_back(P);
       }
     };
@@ -2837,7 +2856,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
             P.lds = 0;
             P.ngroups = 0;
             P.pass_off = b->host_passes.size();
-            P.tile_off = b->host_tiles.size();
+            std::vector<uint32_t> starts;
             uint32_t tiles = 0;
             for (size_t g = 0; g < per_group.size(); ++g) {
               auto& L = per_group[g];
@@ -2847,7 +2866,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
               LinePass lp = L[ph].lp;
               lp.xcd_remap = 0;
               b->host_passes.push_back(lp);
-              b->host_tiles.push_back(tiles);
+              starts.push_back(tiles);
               tiles += (uint32_t)L[ph].grid;
               P.lds = std::max(P.lds, L[ph].lds);
               ++P.ngroups;
@@ -2858,6 +2877,7 @@ static fftahip_status batch_build(fftahip_batch b, hipStream_t s) {
             if (P.ngroups == 0) continue;
             P.grid = tiles;
             P.pass = (int)ph;
+            P.tile_off = tile_map(starts, tiles);
             b->phases.push_back(P);
           }
     };
