@@ -607,20 +607,43 @@ static int64_t row_max(int prec) {
 
 // Bluestein length M = M1 * M2 for the three fused passes: column passes of
 // length M1 (strided, <= pass_max) and one row pass of length M2 (<= row_max).
+// Relative work of a three-pass Bluestein row pass over rows of m2 points
+// with schedule `row` (per point: its stages plus about three for the column
+// passes and the row pass's own load, spectrum and store phases), rows that
+// are not a power of two weighted by FFTAHIP_BLUE_NP2 percent (their
+// radix-3/5 stages and partly filled workgroups ran at 1.4-2.2 TB/s against
+// 3.0 TB/s for 4096-point rows) and rows that need more than 512 threads by
+// FFTAHIP_BLUE_WIDE percent (a 1024-thread workgroup of the full variant
+// needs a nearly idle CU and waits behind the step's other launches).
+static double row_cost(const Inner& row, int64_t m2) {
+  double c = (double)row.rad.size() + 3.0;
+  if (m2 & (m2 - 1)) c *= (double)env_i64("FFTAHIP_BLUE_NP2", 150) / 100.0;
+  if (row.Tl > 512) c *= (double)env_i64("FFTAHIP_BLUE_WIDE", 150) / 100.0;
+  return c;
+}
+
 static bool blue3_split(int64_t M, int prec, int64_t* M1, int64_t* M2) {
   const int64_t pmax = pass_max(prec);
   Inner a, b;
-  // first choice: the longest columns that run as register line passes
+  // first choice: columns that run as register line passes, the row of least
+  // row_cost (ties to the longest columns)
   const int64_t rmax = env_i64("FFTAHIP_REG", 1) != 0 ? reg_pass_max(prec, 2) : 0;
+  double best = 1e300;
   for (int64_t m1 = rmax; m1 >= 8; --m1) {
     if (M % m1) continue;
     const int64_t m2 = M / m1;
     if (m2 > row_max(prec)) break;
+    // the rule's schedules (not the tuned table): the split must not change
+    // after the plan's spectrum rows are laid out for it
     if (!make_inner(m2, b) || !make_inner(m1, a)) continue;
-    *M1 = m1;
-    *M2 = m2;
-    return true;
+    const double cost = row_cost(b, m2);
+    if (cost < best) {
+      best = cost;
+      *M1 = m1;
+      *M2 = m2;
+    }
   }
+  if (best < 1e300) return true;
   // else LDS column passes of >= 32 points (enough work per strided tile),
   // then the longest row that fits
   for (int64_t min_m1 : {(int64_t)32, (int64_t)2}) {
@@ -1785,13 +1808,9 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
     int64_t chosen = best_line_length(N, prec);
     int64_t first = 0;
     // three fused passes: among the splittable 5-smooth M up to 15% above
-    // the first, the least M * (row stages + 3) (the column passes and the
-    // row pass's own load, spectrum and store phases cost about three
-    // stages per element), rows that are not a power of two weighted by
-    // 1.5 (their radix-3/5 stages and partly filled workgroups ran at
-    // 1.4-2.2 TB/s against 3.0 TB/s for 4096-point rows; 3.56 against
-    // 3.61 ms per batched step); env FFTAHIP_BLUE_COST=0 takes the first,
-    // FFTAHIP_BLUE_NP2 sets the weight in percent
+    // the first, the least M * row_cost of the split blue3_split takes
+    // (3.56 against 3.61 ms per batched step with the power-of-two weight);
+    // env FFTAHIP_BLUE_COST=0 takes the first
     const bool costed = env_i64("FFTAHIP_BLUE_COST", 1) != 0;
     double best_cost = 1e300;
     int64_t limit = chosen ? 0 : 4 * (2 * N - 1);  // one kernel: no three-pass search
@@ -1807,8 +1826,7 @@ static fftahip_status init_length(LengthPlan& lp, int prec, int64_t N, int64_t c
       }
       Inner row;
       make_inner(b, row, prec);
-      double cost = (double)M * ((double)row.rad.size() + 3.0);
-      if (b & (b - 1)) cost *= (double)env_i64("FFTAHIP_BLUE_NP2", 150) / 100.0;
+      const double cost = (double)M * row_cost(row, b);
       if (cost < best_cost) {
         best_cost = cost;
         chosen = M;
