@@ -247,6 +247,18 @@ def test_schedule_batch_uneven():
         off += n
 
 
+def test_three_pass_rows_avoid_wide_workgroups():
+    """The three-pass Bluestein row cost (plan.cpp row_cost) weights rows
+    that need 1024-thread workgroups: N = 27779 runs 15 x 4096 (power-of-two
+    rows at 256 threads), not 9 x 6561, and the rows stay within 512 threads
+    for lengths across the three-pass range."""
+    s = P.debug_schedule((27779,), 32, -1)
+    assert [(la["L"], la["mid_mode"]) for la in s["launches"]] == [(15, 0), (4096, 1), (15, 0)]
+    for n in (8209, 12011, 20011, 27779, 40009, 49195, 65521):
+        rows = [la for la in P.debug_schedule((n,), 32, -1)["launches"] if la["mid_mode"] == 1 and not la["reg"]]
+        assert rows and all(la["block"] <= 512 for la in rows), (n, [(la["L"], la["block"]) for la in rows])
+
+
 # Bluestein lines of 2N-1 <= L points: the first sweep skips the zero tail,
 # the second computes only the positions the chirp store reads
 ZERO_PAD_LENGTHS = [457, 4099, 2 * 7919, 3 * 4099, 1031, 2053]
